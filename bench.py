@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""ST reconstruction throughput on one MI355X.
+
+Workload (default): space-time reconstruction of 8 frames of a 256^3 fp32
+volume, 45 golden-angle views per frame on a 512x512 detector (sid 600 mm,
+sdd 1000 mm, 1 mm pitch), primal-dual sigma = tau = 0.2, SART alpha = 0.3,
+lambda = 0.01, k = 0.1, e = 0.05, huber = 0.01, voxel-driven BP, clamped
+cubic warp.  Synthetic data: seeded ellipsoid phantom deformed by a smooth
+sinusoidal field; projections are the forward projections of the phantom.
+
+One step = one ST iteration (a primal-dual pass over all frames: per frame
+dual with backward warp, primal with forward warp, a full SART sweep over the
+frame's views, extrapolation), run in the batched form captured in a hipGraph.
+All inputs are resident in HBM before the timed region.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ST recon iters/sec at 256^3 x 8 frames; projector % HBM peak; vs CPU"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+CONFIGS = {
+    "st256": dict(vol=(256, 256, 256), det=(512, 512), F=8, P=45, amp=4.0),
+    "small": dict(vol=(64, 64, 64), det=(128, 128), F=2, P=8, amp=2.0),
+}
+
+
+def setup(stt, cfg):
+    vol, det, F, P = cfg["vol"], cfg["det"], cfg["F"], cfg["P"]
+    g = stt.Geometry.make(vol, det, (1.0, 1.0, 1.0), 1.0, 600.0, 1000.0)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    p.bp = stt.STT_BP_VOXEL
+    p.interp = stt.STT_INTERP_CUBIC_CLAMPED
+    ctx = stt.Context(g, p, [P] * F)
+    angles = [stt.synth_angles(1, P, offset=P * f) for f in range(F)]
+    projections = []
+    for f in range(F):
+        t0 = time.time()
+        truth = stt.synth_phantom_frame(vol, f, n_ellipsoids=10, seed=1, deform_seed=2, amp=cfg["amp"])
+        ctx.set_views(f, angles[f])
+        ctx.set_volume(f, truth)
+        b = np.empty((P, det[0] * det[1]), np.float32)
+        for k in range(P):
+            b[k] = ctx.forward_project(f, k)[0]
+        ctx.set_projections(f, b)
+        projections.append(b)
+        if f < F - 1:
+            ctx.set_flow(f, stt.synth_flow(vol, f, deform_seed=2, amp=cfg["amp"]))
+        log(f"setup: frame {f} phantom + {P} projections ({time.time() - t0:.1f}s)")
+    zero = np.zeros(int(np.prod(vol)), np.float32)
+    for f in range(F + 1):
+        ctx.set_volume(f, zero)
+    ctx.pd_init()
+    return g, p, ctx, angles, projections
+
+
+def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
+    """Times the CPU oracle on one frame's PD step + nviews SART views and
+    scales to one ST iteration (F frames x P views)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    vol, F, P = cfg["vol"], cfg["F"], cfg["P"]
+    oc = O.OracleCtx(g, p, [P, P])
+    for f in range(2):
+        oc.set_views(f, angles[f])
+        oc.set_projections(f, projections[f].reshape(-1))
+    import spacetimetomography_amd as stt
+    oc.set_flow(0, stt.synth_flow(vol, 0, deform_seed=2, amp=cfg["amp"]))
+    zero = np.zeros(int(np.prod(vol)), np.float32)
+    for f in range(3):
+        oc.set_volume(f, zero)
+    oc.pd_init()
+    t_pd, t_v = C.c_double(), C.c_double()
+    rc = O.orc.o_ctx_time_sample(oc.h, 1, nviews, int(faithful), C.byref(t_pd), C.byref(t_v))
+    assert rc == 0
+    per_view = t_v.value / nviews
+    t_iter = F * (t_pd.value + P * per_view)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": 1.0 / t_iter,
+        "unit": "iters/s",
+        "cores": threads,
+        "kind": "port",
+        "variant": "reference-faithful" if faithful else "lean",
+        "sample": (f"frame 1 of {F}: PD dual+primal with both warps ({t_pd.value:.2f}s) and {nviews} of {P} "
+                   f"SART views ({per_view:.2f}s/view), scaled x{F} frames x{P} views"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="st256", choices=sorted(CONFIGS))
+    ap.add_argument("--form", default="batched-graph", choices=["batched-graph", "batched", "plain"])
+    ap.add_argument("--cpu-views", type=int, default=3, help="SART views in the CPU-baseline sample")
+    ap.add_argument("--cpu-lean", action="store_true", help="also time the lean CPU variant")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(frame-sharded multi-GPU is a design, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import spacetimetomography_amd as stt
+
+    cfg = CONFIGS[args.config]
+    t0 = time.time()
+    g, p, ctx, angles, projections = setup(stt, cfg)
+    log(f"setup done in {time.time() - t0:.1f}s")
+    batched = args.form != "plain"
+    use_graph = args.form == "batched-graph"
+
+    ctx.iterate(args.warmup, batched=batched, use_graph=use_graph)
+    ctx.synchronize()
+    log(f"warmup: {args.warmup} iterations")
+    t_start = time.perf_counter()
+    ctx.iterate(args.steps, batched=batched, use_graph=use_graph)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    log(f"timed: {args.steps} iterations in {elapsed:.3f}s")
+
+    # Per-kernel durations from HIP events on the context stream (eager launches).
+    ksteps = max(1, min(args.steps, 3))
+    fp_ms = bp_ms = dual_ms = primal_ms = 0.0
+    fp_n = bp_n = 0
+    for _ in range(ksteps):
+        t = ctx.iterate(1, batched=batched, use_graph=False, timing=True)
+        fp_ms += t.fp_ms
+        fp_n += t.fp_launches
+        bp_ms += t.bp_ms
+        bp_n += t.bp_launches
+        dual_ms += t.dual_ms
+        primal_ms += t.primal_ms
+    ctx.synchronize()
+    log(f"kernel timing: {ksteps} eager iterations")
+
+    vol, det, F, P = cfg["vol"], cfg["det"], cfg["F"], cfg["P"]
+    V = int(np.prod(vol))
+    D = det[0] * det[1]
+    frames_per_fp = F if batched else 1
+    fp_avg_s = fp_ms / max(fp_n, 1) / 1e3
+    fp_bytes = frames_per_fp * (4 * V + 8 * D)  # volume read once + b read + correction write
+    achieved = fp_bytes / fp_avg_s
+    iter_bytes = F * P * (12 * V + 12 * D) + F * 124 * V
+    ms_per_step = elapsed / args.steps * 1e3
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded deformed ellipsoid phantom, projections = its forward projections)",
+        "config": {
+            "workload": (f"space-time PD+SART reconstruction {vol[0]}x{vol[1]}x{vol[2]} fp32 x {F} frames, "
+                         f"{P} golden-angle views/frame, {det[0]}x{det[1]} detector, voxel BP, clamped-cubic warp"),
+            "form": args.form,
+            "volume": list(vol), "frames": F, "views_per_frame": P, "detector": list(det),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_fp<SART> (forward projection + fused SART correction)",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "launch_avg_ms": fp_avg_s * 1e3,
+            "bytes_per_launch": fp_bytes,
+        },
+        "kernel_ms_per_step": {
+            "fp": fp_ms / ksteps, "bp": bp_ms / ksteps, "pd_dual": dual_ms / ksteps, "pd_primal": primal_ms / ksteps,
+        },
+        "iteration_hbm_frac": iter_bytes / (ms_per_step / 1e3) / HBM_PEAK,
+    }
+    if not args.no_cpu_baseline:
+        t1 = time.time()
+        out["cpu_baseline"] = cpu_baseline(g, p, cfg, angles, projections, args.cpu_views, faithful=True)
+        log(f"cpu baseline (faithful) {out['cpu_baseline']['value']:.5f} it/s in {time.time() - t1:.1f}s")
+        if args.cpu_lean:
+            out["cpu_baseline_lean"] = cpu_baseline(g, p, cfg, angles, projections, args.cpu_views, faithful=False)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,233 @@
+/*
+ * stt.h — C-ABI of the MI355X space-time tomography hot path.
+ *
+ * The reference program (gmzang/SpaceTimeTomography) exposes this path only as
+ * in-process C++ classes (ST_Tomography, TypicalRayProducer, RayCaster,
+ * VolumeData, warping::warp).  Each entry point below replaces one of those
+ * members or free functions; the citation next to it names the reference code
+ * whose behaviour it reproduces.  Plain pointers and sizes only: no torch or
+ * HIP types cross this boundary.
+ *
+ * Conventions
+ *   - Every function returns an int status (STT_OK on success).  On failure
+ *     stt_last_error() returns a message for the calling thread.  No C++
+ *     exception crosses the ABI.
+ *   - Volumes are planar fp32, x fastest, then y, then z (CImg layout,
+ *     CImg.h:11159-11161).  A flow field is three such planes (u, v, w) in
+ *     voxel units.  Projections are W x H x P fp32, pixel (i, j) of view p at
+ *     i + j*W + p*W*H (the frame's slice of the projection stack,
+ *     ST_Tomography.cpp:355).
+ *   - Host pointers are copied in or out; "_dev" variants take device pointers
+ *     that the library borrows and never frees.
+ *   - One context per thread.  All device work of a context is issued on the
+ *     context's own HIP stream; functions that return data synchronise it.
+ */
+#ifndef STT_H
+#define STT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STT_OK            0
+#define STT_ERR_INVALID   1  /* null pointer, out-of-range index, size mismatch */
+#define STT_ERR_HIP       2  /* HIP runtime error */
+#define STT_ERR_NOMEM     3  /* device or host allocation failed */
+#define STT_ERR_STATE     4  /* call out of order (e.g. no views or projections set) */
+#define STT_ERR_NODEVICE  5  /* no HIP device visible */
+
+/* Backprojector (ST_Tomography.cpp:221-226, main.cpp:43 "--bp"). */
+#define STT_BP_VOXEL      0  /* voxel-driven, bicubic detector sampling (ST_Tomography.cpp:687-731) */
+#define STT_BP_RAY        1  /* ray-driven scatter + normalise (ST_Tomography.cpp:733-765) */
+
+/* Spatial prior (main.cpp:42 "--prior").  The reference only implements ATV
+ * (per-component clamp, ST_Tomography.cpp:523-525).  STV is this library's
+ * isotropic spatio-temporal TV (4-component gradient incl. the forward
+ * temporal difference, projected onto the lambda L2 ball). */
+#define STT_PRIOR_ATV     0
+#define STT_PRIOR_TV      1  /* isotropic spatial TV: 3-component L2-ball projection */
+#define STT_PRIOR_STV     2
+
+/* Interpolation of the semi-Lagrangian warp (warping.h:49-88). */
+#define STT_INTERP_CUBIC_CLAMPED 0  /* CImg Catmull-Rom clamped to the 8-corner min/max (reference) */
+#define STT_INTERP_TRILINEAR     1  /* trilinear, same trace; differentiable path */
+
+/* Scan and volume geometry: the geometric part of the ST_Tomography
+ * constructor parameter set (ST_Tomography.h:27-29, main.cpp:26-37). */
+typedef struct stt_geometry {
+    int    vol[3];      /* volume size in voxels (main.cpp -f, "volxyz") */
+    float  spacing[3];  /* voxel spacing in mm (main.cpp -c; VolumeData::m_Spacings) */
+    int    det[2];      /* detector W, H in pixels (main.cpp -u, "imgwh") */
+    double ds;          /* detector pitch in mm (main.cpp -d) */
+    double sid;         /* source to iso-centre distance in mm (main.cpp -g) */
+    double sdd;         /* source to detector distance in mm (main.cpp -j) */
+    double oxyz[3];     /* volume-centre offset in mm (main.cpp --oxyz; ST_Tomography.cpp:244-246) */
+} stt_geometry;
+
+/* Solver parameters: the numerical part of the constructor parameter set.
+ * stt_params_reference() fills the values the reference actually runs with
+ * (its hard-coded globals, ST_Tomography.cpp:93-114, 219). */
+typedef struct stt_params {
+    double sigma;        /* primal-dual sigma (CP_sigma, ST_Tomography.cpp:102) */
+    double tau;          /* primal-dual tau (CP_tau, :100); SART factor = sqrt(tau) (:110-111) */
+    double lambda_tv;    /* TV weight (TV_w, main.cpp -l) */
+    double k_tp;         /* temporal-prior weight (TP_w, main.cpp -k) */
+    double e_bc;         /* brightness-constancy weight (BC_w, main.cpp -e) */
+    double huber;        /* huber epsilon (main.cpp --huber; m_huberfactor :219) */
+    double alpha;        /* SART step (main.cpp -a) */
+    float  g_max;        /* upper clamp; <= 0 means max(frame-0 projections)/4 (:356-357) */
+    int    prior;        /* STT_PRIOR_* */
+    int    bp;           /* STT_BP_* */
+    int    interp;       /* STT_INTERP_* for the PD warps */
+    int    roi_cut[8];   /* region of interest cut[] (:114); all zero = whole interior */
+    int    roi_jtop;     /* the constant 7 in imROI's j test (:324); 0 = none */
+    int    fp64_reductions; /* 1: residual norms and G_Max reduction in FP64 */
+} stt_params;
+
+/* Per-kernel timing of the most recent stt_iterate (HIP events on the
+ * context stream).  Times in milliseconds, summed over launches. */
+typedef struct stt_timing {
+    double total_ms;
+    double fp_ms;        int fp_launches;
+    double bp_ms;        int bp_launches;
+    double dual_ms;      int dual_launches;
+    double primal_ms;    int primal_launches;
+} stt_timing;
+
+typedef struct stt_ctx stt_ctx;
+
+/* ---- library / errors --------------------------------------------------- */
+const char* stt_last_error(void);
+int  stt_version(void);
+/* Fills p with the reference's effective values (sigma=0.1125, tau=1,
+ * huber=0.1, alpha=0.1, lambda=0.1, k=e=0 from main.cpp:67-95 and
+ * ST_Tomography.cpp:99-114, rose ROI cut[]).  Pure host function. */
+void stt_params_reference(stt_params* p);
+/* Same but with the whole-interior ROI (cut = 0, jtop = 0). */
+void stt_params_default(stt_params* p);
+
+/* ---- context (ST_Tomography ctor + SetData, ST_Tomography.cpp:200-361) -- */
+/* nframes volumes plus one padding copy of the last frame (VolumeData.cpp:126);
+ * views_per_frame[f] views for frame f (uneven counts allowed). */
+int  stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes,
+                    const int* views_per_frame, stt_ctx** out);
+int  stt_ctx_destroy(stt_ctx* ctx);
+int  stt_ctx_set_params(stt_ctx* ctx, const stt_params* p);
+int  stt_ctx_get_gmax(stt_ctx* ctx, float* g_max);
+
+/* View tables.  Angles in degrees about +y (InitViews, ST_Tomography.cpp:887-934):
+ * each view is Quaternion(0,1,0, theta*3.1415926/180). */
+int  stt_set_views(stt_ctx* ctx, int frame, const float* angles_deg, int n);
+/* Explicit rotations, x,y,z,w per view (pQuaternion[frame][view]). */
+int  stt_set_view_quaternions(stt_ctx* ctx, int frame, const float* q_xyzw, int n);
+
+/* Data (VolumeData::m_Volumelist / m_Flowlist, Projslist). frame in [0, nframes];
+ * frame == nframes is the padding copy. */
+int  stt_set_volume(stt_ctx* ctx, int frame, const float* vol);
+int  stt_get_volume(stt_ctx* ctx, int frame, float* vol);
+int  stt_set_flow(stt_ctx* ctx, int pair, const float* uvw);       /* pair in [0, nframes-1] */
+int  stt_set_projections(stt_ctx* ctx, int frame, const float* b); /* W*H*views[frame] */
+/* Device-pointer variants (borrowed, copied device-to-device). */
+int  stt_set_volume_dev(stt_ctx* ctx, int frame, const float* vol_dev);
+int  stt_get_volume_dev(stt_ctx* ctx, int frame, float* vol_dev);
+
+/* Dual / auxiliary state readout (Y_tv 3 planes, Y_tp, Y_bc, X_bar, X_prev). */
+#define STT_BUF_XBAR  0
+#define STT_BUF_XPREV 1
+#define STT_BUF_YTV   2
+#define STT_BUF_YTP   3
+#define STT_BUF_YBC   4
+int  stt_get_buffer(stt_ctx* ctx, int which, int frame, float* out);
+int  stt_set_buffer(stt_ctx* ctx, int which, int frame, const float* in);
+
+/* ---- ImageFormationModel operators -------------------------------------- */
+/* Forward projection of view `view` of frame `frame` from the current volume
+ * (GenerateImage_GetCorrections -> GenerateImage_MT -> ProduceRay ->
+ * CastingRay, TypicalRayProducer.core.h:32-69,123-263, RayCaster.core.h:42-187).
+ * Ax: line integrals; raylen: tfar-tnear for hit rays, left unchanged for
+ * missed rays (the reference keeps the previous value); weight: sample
+ * counts.  Any output pointer may be NULL.  raylen is read and written. */
+int  stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax,
+                         float* raylen, float* weight);
+/* Voxel-driven backprojection of a correction image (ST_Tomography.cpp:687-731):
+ * x += alpha*bicubic(corr), clamp [0, G_Max]. */
+int  stt_backproject_voxel(stt_ctx* ctx, int frame, int view, const float* corr,
+                           double alpha);
+/* Ray-driven backprojection (ST_Tomography.cpp:733-765 with the scatter of
+ * SingleScalarVolume.core.h:128-143): x += contrib/wsum, clamp (0/0 -> 0).
+ * contrib/wsum may be NULL; when given they receive the accumulators. */
+int  stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr,
+                         float* contrib, float* wsum);
+/* Matched transpose of stt_forward_project: out += A^T y (the exact adjoint of
+ * the sampled line integral including the sd factor).  VJP of the projector
+ * with respect to the volume. */
+int  stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out);
+
+/* One SART sweep of frame `frame` (ST_Tomography::SART, :616-808):
+ * views in order[0..n) (n <= views[frame]; NULL = identity order), then
+ * x *= imROI. */
+int  stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n);
+
+/* ---- ST_Tomo proximal steps --------------------------------------------- */
+/* Semi-Lagrangian warp warping::warp(dim, aabb, vel, src, dt) (warping.h:116-145)
+ * on the context's volume grid; host pointers; aabb = [0, dims]. */
+int  stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt,
+              float* out, int interp);
+/* VJPs of the warp: given g = dL/d(out), accumulate dL/d(src) into gsrc and
+ * dL/d(uvw) into guvw (either may be NULL).  interp must be TRILINEAR. */
+int  stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt,
+                  const float* g, float* gsrc, float* guvw);
+
+/* Initialise X_bar <- x for all frames and zero the duals
+ * (ST_Tomography.cpp:494-500 first pass, SetData :339-350). */
+int  stt_pd_init(stt_ctx* ctx);
+/* Dual step of one frame (ST_Tomography.cpp:502-534). */
+int  stt_pd_dual(stt_ctx* ctx, int frame);
+/* Primal step of one frame (ST_Tomography.cpp:536-578). */
+int  stt_pd_primal(stt_ctx* ctx, int frame);
+/* X_bar = 2x - X_prev on the ROI (ST_Tomography.cpp:586-604). */
+int  stt_pd_extrapolate(stt_ctx* ctx, int frame);
+
+/* One or more ST iterations (one "No_PO" pass over all frames,
+ * ST_Tomography.cpp:477-608): per frame dual, primal, SART sweep over all of
+ * the frame's views in view-table order, extrapolation.
+ *   batched = 0: the reference's per-frame order, one frame per launch.
+ *   batched = 1: all duals, then all primals, then every frame's view k in one
+ *                launch (k = 0..max views); identical results.
+ *   use_graph = 1: capture one iteration in a hipGraph and replay it.
+ * timing may be NULL; when given, kernels are bracketed with HIP events. */
+int  stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph,
+                 stt_timing* timing);
+/* Blocks until the context stream is idle. */
+int  stt_synchronize(stt_ctx* ctx);
+
+/* FP64 reductions on the device: sum of squares of the data residual
+ * b - A x over all frames and views (one forward projection per view). */
+int  stt_residual_norm(stt_ctx* ctx, double* r2);
+/* Power iteration for the operator norm of the PD coupling operator
+ * (spatial forward difference stacked with the temporal difference). */
+int  stt_power_iteration(stt_ctx* ctx, int iters, double* L);
+
+/* ---- host-side synthetic inputs (no device needed) ----------------------- */
+/* Additive ellipsoid phantom on an nx*ny*nz grid (seeded). */
+int  stt_synth_phantom(int nx, int ny, int nz, int n_ellipsoids, uint64_t seed,
+                       float* out);
+/* The same phantom pulled back through the smooth deformation D(x,t) of
+ * frame t (sum of 4 sinusoidal modes, max |D| <= amp voxels). */
+int  stt_synth_phantom_frame(int nx, int ny, int nz, int n_ellipsoids,
+                             uint64_t seed, uint64_t deform_seed, float amp,
+                             int t, float* out);
+/* Flow of frame pair (t, t+1): u = D(x, t+1) - D(x, t), three planes. */
+int  stt_synth_flow(int nx, int ny, int nz, uint64_t deform_seed, float amp,
+                    int t, float* uvw);
+/* View tables (degrees): kind 0 = even over 360, 1 = golden angle
+ * ((offset+k)*180*(3-sqrt 5)) mod 360, 2 = van der Corput base 2 scaled to 360. */
+int  stt_synth_angles(int kind, int n, int offset, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STT_H */

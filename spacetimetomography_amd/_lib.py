@@ -1,0 +1,296 @@
+"""ctypes binding of the C-ABI declared in include/stt.h.
+
+The shared library lives in-tree at spacetimetomography_amd/lib/libstt.so and
+is built by spacetimetomography_amd/build.py.  There is no fallback: if the
+library is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libstt.so")
+
+STT_OK = 0
+STT_ERR_INVALID = 1
+STT_ERR_HIP = 2
+STT_ERR_NOMEM = 3
+STT_ERR_STATE = 4
+STT_ERR_NODEVICE = 5
+
+STT_BP_VOXEL, STT_BP_RAY = 0, 1
+STT_PRIOR_ATV, STT_PRIOR_TV, STT_PRIOR_STV = 0, 1, 2
+STT_INTERP_CUBIC_CLAMPED, STT_INTERP_TRILINEAR = 0, 1
+STT_BUF_XBAR, STT_BUF_XPREV, STT_BUF_YTV, STT_BUF_YTP, STT_BUF_YBC = range(5)
+
+
+class Geometry(C.Structure):
+    """stt_geometry: vol, spacing, det, ds, sid, sdd, oxyz (main.cpp flag set)."""
+    _fields_ = [("vol", C.c_int * 3), ("spacing", C.c_float * 3), ("det", C.c_int * 2),
+                ("ds", C.c_double), ("sid", C.c_double), ("sdd", C.c_double),
+                ("oxyz", C.c_double * 3)]
+
+    @classmethod
+    def make(cls, vol, det, spacing=(1.0, 1.0, 1.0), ds=1.0, sid=600.0, sdd=1000.0,
+             oxyz=(0.0, 0.0, 0.0)):
+        g = cls()
+        g.vol[:] = [int(v) for v in vol]
+        g.spacing[:] = [float(s) for s in spacing]
+        g.det[:] = [int(d) for d in det]
+        g.ds, g.sid, g.sdd = float(ds), float(sid), float(sdd)
+        g.oxyz[:] = [float(o) for o in oxyz]
+        return g
+
+
+class Params(C.Structure):
+    """stt_params (the numerical part of the ST_Tomography constructor)."""
+    _fields_ = [("sigma", C.c_double), ("tau", C.c_double), ("lambda_tv", C.c_double),
+                ("k_tp", C.c_double), ("e_bc", C.c_double), ("huber", C.c_double),
+                ("alpha", C.c_double), ("g_max", C.c_float), ("prior", C.c_int),
+                ("bp", C.c_int), ("interp", C.c_int), ("roi_cut", C.c_int * 8),
+                ("roi_jtop", C.c_int), ("fp64_reductions", C.c_int)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double),
+                ("fp_ms", C.c_double), ("fp_launches", C.c_int),
+                ("bp_ms", C.c_double), ("bp_launches", C.c_int),
+                ("dual_ms", C.c_double), ("dual_launches", C.c_int),
+                ("primal_ms", C.c_double), ("primal_launches", C.c_int)]
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libstt.so not built at {LIB_PATH}; run spacetimetomography_amd/build.py")
+
+lib = C.CDLL(LIB_PATH)
+
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
+
+_SIGS = {
+    "stt_last_error": (C.c_char_p, []),
+    "stt_version": (C.c_int, []),
+    "stt_params_reference": (None, [C.POINTER(Params)]),
+    "stt_params_default": (None, [C.POINTER(Params)]),
+    "stt_ctx_create": (C.c_int, [C.POINTER(Geometry), C.POINTER(Params), C.c_int, _I, C.POINTER(_P)]),
+    "stt_ctx_destroy": (C.c_int, [_P]),
+    "stt_ctx_set_params": (C.c_int, [_P, C.POINTER(Params)]),
+    "stt_ctx_get_gmax": (C.c_int, [_P, _F]),
+    "stt_set_views": (C.c_int, [_P, C.c_int, _F, C.c_int]),
+    "stt_set_view_quaternions": (C.c_int, [_P, C.c_int, _F, C.c_int]),
+    "stt_set_volume": (C.c_int, [_P, C.c_int, _F]),
+    "stt_get_volume": (C.c_int, [_P, C.c_int, _F]),
+    "stt_set_flow": (C.c_int, [_P, C.c_int, _F]),
+    "stt_set_projections": (C.c_int, [_P, C.c_int, _F]),
+    "stt_set_volume_dev": (C.c_int, [_P, C.c_int, _P]),
+    "stt_get_volume_dev": (C.c_int, [_P, C.c_int, _P]),
+    "stt_get_buffer": (C.c_int, [_P, C.c_int, C.c_int, _F]),
+    "stt_set_buffer": (C.c_int, [_P, C.c_int, C.c_int, _F]),
+    "stt_forward_project": (C.c_int, [_P, C.c_int, C.c_int, _F, _F, _F]),
+    "stt_backproject_voxel": (C.c_int, [_P, C.c_int, C.c_int, _F, C.c_double]),
+    "stt_backproject_ray": (C.c_int, [_P, C.c_int, C.c_int, _F, _F, _F]),
+    "stt_fp_vjp": (C.c_int, [_P, C.c_int, C.c_int, _F, _F]),
+    "stt_sart_sweep": (C.c_int, [_P, C.c_int, _I, C.c_int]),
+    "stt_warp": (C.c_int, [_P, _F, _F, C.c_float, _F, C.c_int]),
+    "stt_warp_vjp": (C.c_int, [_P, _F, _F, C.c_float, _F, _F, _F]),
+    "stt_pd_init": (C.c_int, [_P]),
+    "stt_pd_dual": (C.c_int, [_P, C.c_int]),
+    "stt_pd_primal": (C.c_int, [_P, C.c_int]),
+    "stt_pd_extrapolate": (C.c_int, [_P, C.c_int]),
+    "stt_iterate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Timing)]),
+    "stt_synchronize": (C.c_int, [_P]),
+    "stt_residual_norm": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "stt_power_iteration": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    "stt_synth_phantom": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _F]),
+    "stt_synth_phantom_frame": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+                                          C.c_float, C.c_int, _F]),
+    "stt_synth_flow": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_int, _F]),
+    "stt_synth_angles": (C.c_int, [C.c_int, C.c_int, C.c_int, _F]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+class SttError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"stt error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != STT_OK:
+        raise SttError(rc, lib.stt_last_error().decode(errors="replace"))
+
+
+def fptr(a):
+    """float* of a C-contiguous float32 numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "need a contiguous float32 array"
+    return a.ctypes.data_as(_F)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_I)
+
+
+def params_default():
+    p = Params()
+    lib.stt_params_default(C.byref(p))
+    return p
+
+
+def params_reference():
+    p = Params()
+    lib.stt_params_reference(C.byref(p))
+    return p
+
+
+def synth_phantom(dims, n_ellipsoids=10, seed=1):
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, np.float32)
+    check(lib.stt_synth_phantom(nx, ny, nz, n_ellipsoids, seed, fptr(out)))
+    return out
+
+
+def synth_phantom_frame(dims, t, n_ellipsoids=10, seed=1, deform_seed=2, amp=4.0):
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, np.float32)
+    check(lib.stt_synth_phantom_frame(nx, ny, nz, n_ellipsoids, seed, deform_seed, amp, t, fptr(out)))
+    return out
+
+
+def synth_flow(dims, t, deform_seed=2, amp=4.0):
+    nx, ny, nz = dims
+    out = np.empty(3 * nx * ny * nz, np.float32)
+    check(lib.stt_synth_flow(nx, ny, nz, deform_seed, amp, t, fptr(out)))
+    return out
+
+
+def synth_angles(kind, n, offset=0):
+    out = np.empty(n, np.float32)
+    check(lib.stt_synth_angles(kind, n, offset, fptr(out)))
+    return out
+
+
+class Context:
+    """Thin owner of an stt_ctx (one HIP stream, all solver state in HBM)."""
+
+    def __init__(self, geometry, params, views_per_frame):
+        self.g = geometry
+        self.p = params
+        self.P = np.ascontiguousarray(views_per_frame, dtype=np.int32)
+        self.F = len(self.P)
+        self.V = int(np.prod(list(geometry.vol)))
+        self.D = int(geometry.det[0] * geometry.det[1])
+        h = _P()
+        check(lib.stt_ctx_create(C.byref(geometry), C.byref(params), self.F, iptr(self.P), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.stt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_views(self, frame, angles_deg):
+        a = np.ascontiguousarray(angles_deg, np.float32)
+        check(lib.stt_set_views(self.h, frame, fptr(a), len(a)))
+
+    def set_view_quaternions(self, frame, q):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1)
+        check(lib.stt_set_view_quaternions(self.h, frame, fptr(q), len(q) // 4))
+
+    def set_volume(self, frame, vol):
+        check(lib.stt_set_volume(self.h, frame, fptr(np.ascontiguousarray(vol, np.float32).reshape(-1))))
+
+    def get_volume(self, frame):
+        out = np.empty(self.V, np.float32)
+        check(lib.stt_get_volume(self.h, frame, fptr(out)))
+        return out
+
+    def set_flow(self, pair, uvw):
+        check(lib.stt_set_flow(self.h, pair, fptr(np.ascontiguousarray(uvw, np.float32).reshape(-1))))
+
+    def set_projections(self, frame, b):
+        check(lib.stt_set_projections(self.h, frame, fptr(np.ascontiguousarray(b, np.float32).reshape(-1))))
+
+    def get_buffer(self, which, frame):
+        n = 3 * self.V if which == STT_BUF_YTV else self.V
+        out = np.empty(n, np.float32)
+        check(lib.stt_get_buffer(self.h, which, frame, fptr(out)))
+        return out
+
+    def gmax(self):
+        g = C.c_float()
+        check(lib.stt_ctx_get_gmax(self.h, C.byref(g)))
+        return g.value
+
+    def forward_project(self, frame, view, raylen=None):
+        ax = np.empty(self.D, np.float32)
+        rl = np.zeros(self.D, np.float32) if raylen is None else np.ascontiguousarray(raylen, np.float32).copy()
+        w = np.empty(self.D, np.float32)
+        check(lib.stt_forward_project(self.h, frame, view, fptr(ax), fptr(rl), fptr(w)))
+        return ax, rl, w
+
+    def backproject_voxel(self, frame, view, corr, alpha):
+        check(lib.stt_backproject_voxel(self.h, frame, view, fptr(np.ascontiguousarray(corr, np.float32)), alpha))
+
+    def backproject_ray(self, frame, view, corr):
+        c = np.zeros(self.V, np.float32)
+        w = np.zeros(self.V, np.float32)
+        check(lib.stt_backproject_ray(self.h, frame, view, fptr(np.ascontiguousarray(corr, np.float32)),
+                                      fptr(c), fptr(w)))
+        return c, w
+
+    def fp_vjp(self, frame, view, y, out=None):
+        o = np.zeros(self.V, np.float32) if out is None else np.ascontiguousarray(out, np.float32).copy()
+        check(lib.stt_fp_vjp(self.h, frame, view, fptr(np.ascontiguousarray(y, np.float32)), fptr(o)))
+        return o
+
+    def sart_sweep(self, frame, order=None):
+        n = int(self.P[frame]) if order is None else len(order)
+        o = None if order is None else np.ascontiguousarray(order, np.int32)
+        check(lib.stt_sart_sweep(self.h, frame, iptr(o), n))
+
+    def warp(self, uvw, src, dt, interp=STT_INTERP_CUBIC_CLAMPED):
+        out = np.empty(self.V, np.float32)
+        check(lib.stt_warp(self.h, fptr(np.ascontiguousarray(uvw, np.float32)),
+                           fptr(np.ascontiguousarray(src, np.float32)), dt, fptr(out), interp))
+        return out
+
+    def pd_init(self):
+        check(lib.stt_pd_init(self.h))
+
+    def pd_dual(self, frame):
+        check(lib.stt_pd_dual(self.h, frame))
+
+    def pd_primal(self, frame):
+        check(lib.stt_pd_primal(self.h, frame))
+
+    def pd_extrapolate(self, frame):
+        check(lib.stt_pd_extrapolate(self.h, frame))
+
+    def iterate(self, n=1, batched=True, use_graph=False, timing=False):
+        t = Timing() if timing else None
+        check(lib.stt_iterate(self.h, n, int(batched), int(use_graph), C.byref(t) if t is not None else None))
+        return t
+
+    def synchronize(self):
+        check(lib.stt_synchronize(self.h))

@@ -1,0 +1,864 @@
+// stt_capi.cpp — the C-ABI of include/stt.h on one HIP device.
+//
+// A context owns every array of the reference's solver state in HBM
+// (volumes x[0..F], X_bar, X_prev, duals Y_tv/Y_tp/Y_bc, flows, projection
+// stacks, detector buffers) plus a device table of per-view geometry records
+// and pre-built work tables for the two execution forms:
+//   per-frame : the reference's order (ST_Tomography.cpp:492-607), one frame
+//               per launch, one shared stale ray-length buffer (the single
+//               TypicalRayProducer of the reference);
+//   batched   : all duals, all primals, then view k of every frame in one
+//               launch; per-frame ray-length buffers.
+// All work goes to the context's stream; an iteration can be captured in a
+// hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/stt.h"
+#include "stt_device.hpp"
+#include "stt_host.hpp"
+#include "stt_kernels.hpp"
+
+using namespace stt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? STT_ERR_NOMEM : STT_ERR_HIP,                        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                             \
+    } while (0)
+
+// zero tail behind every volume so the projector's +1 trilinear taps need no clamp
+size_t tail_pad(const int* d) {
+    size_t p = (size_t)d[0] * d[1] + d[0] + 2;
+    return (p + 63) / 64 * 64;
+}
+
+}  // namespace
+
+struct stt_ctx {
+    stt_geometry g{};
+    stt_params p{};
+    int F = 0;
+    std::vector<int> P;
+    std::vector<int> view_off;  // prefix sums of P
+    int total_views = 0;
+    int maxP = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    VolGeom vg{};
+    PdConst pc{};
+    size_t V = 0, Vpad = 0, D = 0;
+    float gmax = 10.0f;
+    bool gmax_from_data = true;
+    bool pd_ready = false;
+
+    // device buffers
+    std::vector<float*> x, xbar, xprev, ytv, ytp, ybc, flow, b, raylen_f, corr_f, contrib_f, wsum_f;
+    float* raylen_shared = nullptr;
+    float* zero_flow = nullptr;
+    ViewRec* views_dev = nullptr;
+    std::vector<ViewRec> views_host;
+    std::vector<char> views_set;
+    FrameBufs* fb_dev = nullptr;
+    // work tables: plain (frame-major, one item per view) and batched (per step k)
+    WorkItem* items_plain = nullptr;
+    WorkItem* items_batch = nullptr;
+    std::vector<int> batch_off, batch_cnt;
+    WorkItem* items_scratch = nullptr;  // ad-hoc single launches
+    int scratch_cap = 0;
+    bool tables_dirty = true;
+    // graph
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_batched = -1;
+    std::vector<void*> allocs;
+
+    ~stt_ctx() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        for (void* a : allocs) (void)hipFree(a);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int alloc(float** ptr, size_t nfloats) {
+        void* d = nullptr;
+        HIPCHK(hipMalloc(&d, nfloats * sizeof(float)));
+        allocs.push_back(d);
+        HIPCHK(hipMemsetAsync(d, 0, nfloats * sizeof(float), stream));
+        *ptr = (float*)d;
+        return STT_OK;
+    }
+    template <class T>
+    int alloc_t(T** ptr, size_t n) {
+        void* d = nullptr;
+        HIPCHK(hipMalloc(&d, n * sizeof(T)));
+        allocs.push_back(d);
+        HIPCHK(hipMemsetAsync(d, 0, n * sizeof(T), stream));
+        *ptr = (T*)d;
+        return STT_OK;
+    }
+
+    void derive_consts() {
+        pc.sigma = (float)p.sigma;
+        pc.tau = (float)p.tau;
+        pc.lambda = (float)p.lambda_tv;
+        pc.ktp = (float)p.k_tp;
+        pc.ebc = (float)p.e_bc;
+        // m_huberfactor = 1/(1 + sigma*huber/lambda) (ST_Tomography.cpp:219)
+        pc.huber_factor = (float)(1.0f / (1.0f + (double)pc.sigma * p.huber / p.lambda_tv));
+        // Factor = sqrtf(2*SART_lambda), SART_lambda = tau/2 (:110-111)
+        pc.factor = std::sqrt(2.0f * (pc.tau / 2.0f));
+        pc.alpha = (float)p.alpha;
+        for (int i = 0; i < 8; ++i) pc.cut[i] = p.roi_cut[i];
+        pc.jtop = p.roi_jtop;
+        pc.prior = p.prior;
+        pc.interp = p.interp;
+        pc.nframes = F;
+        if (p.g_max > 0) {
+            gmax = p.g_max;
+            gmax_from_data = false;
+        } else {
+            gmax_from_data = true;
+        }
+        pc.gmax = gmax;
+    }
+
+    WorkItem make_item(int f, int view, bool shared_raylen) const {
+        WorkItem it{};
+        it.view = views_dev + view_off[f] + view;
+        it.vol = x[f];
+        it.b = b[f] + (size_t)view * D;
+        it.raylen = shared_raylen ? raylen_shared : raylen_f[f];
+        it.corr = corr_f[f];
+        it.contrib = contrib_f.empty() ? nullptr : contrib_f[f];
+        it.wsum = wsum_f.empty() ? nullptr : wsum_f[f];
+        it.xprev = xprev[f];
+        it.xbar = xbar[f];
+        it.frame = f;
+        it.last = (view == P[f] - 1) ? 1 : 0;
+        return it;
+    }
+
+    int build_tables() {
+        if (!tables_dirty) return STT_OK;
+        std::vector<WorkItem> plain;
+        for (int f = 0; f < F; ++f)
+            for (int k = 0; k < P[f]; ++k) plain.push_back(make_item(f, k, true));
+        HIPCHK(hipMemcpyAsync(items_plain, plain.data(), plain.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+        std::vector<WorkItem> batch;
+        batch_off.assign(maxP, 0);
+        batch_cnt.assign(maxP, 0);
+        for (int k = 0; k < maxP; ++k) {
+            batch_off[k] = (int)batch.size();
+            for (int f = 0; f < F; ++f)
+                if (k < P[f]) batch.push_back(make_item(f, k, false));
+            batch_cnt[k] = (int)batch.size() - batch_off[k];
+        }
+        HIPCHK(hipMemcpyAsync(items_batch, batch.data(), batch.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+        std::vector<FrameBufs> fbs(F + 1);
+        for (int f = 0; f <= F; ++f) {
+            fbs[f].x = x[f];
+            fbs[f].xbar = xbar[f];
+            fbs[f].xprev = xprev[f];
+            fbs[f].ytv = ytv[f];
+            fbs[f].ytp = ytp[f];
+            fbs[f].ybc = ybc[f];
+            fbs[f].flow = f < F ? flow[f] : zero_flow;
+        }
+        HIPCHK(hipMemcpyAsync(fb_dev, fbs.data(), fbs.size() * sizeof(FrameBufs), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        tables_dirty = false;
+        return STT_OK;
+    }
+
+    int ready_check() const {
+        for (int f = 0; f < F; ++f)
+            if (!views_set[f]) return fail(STT_ERR_STATE, "views of frame " + std::to_string(f) + " not set");
+        return STT_OK;
+    }
+
+    // Ad-hoc launch table (synchronous upload; callers synchronise anyway).
+    int upload_scratch(const std::vector<WorkItem>& items) {
+        if ((int)items.size() > scratch_cap) return fail(STT_ERR_INVALID, "too many work items");
+        HIPCHK(hipMemcpyAsync(items_scratch, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));  // the host vector dies with the caller's expression
+        return STT_OK;
+    }
+
+    int launch_bp_for(const WorkItem* items_dev, int n, bool ray_bp) {
+        if (ray_bp) {
+            HIPCHK(launch_fp(FP_MODE_SCATTER, items_dev, n, vg, 1.0f, stream));
+            HIPCHK(launch_bp_ray_apply(items_dev, n, vg, pc, stream));
+        } else {
+            HIPCHK(launch_bp_voxel(items_dev, n, vg, pc, stream));
+        }
+        return STT_OK;
+    }
+
+    struct Timer {
+        stt_timing* t;
+        hipStream_t s;
+        std::vector<hipEvent_t> ev;
+        std::vector<int> kind;
+        explicit Timer(stt_timing* tt, hipStream_t ss) : t(tt), s(ss) {}
+        ~Timer() {
+            for (auto e : ev) (void)hipEventDestroy(e);
+        }
+        void mark(int k) {
+            if (!t) return;
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            (void)hipEventRecord(e, s);
+            ev.push_back(e);
+            kind.push_back(k);
+        }
+        // kind of interval i = kind recorded at its end mark
+        void collect() {
+            if (!t || ev.size() < 2) return;
+            (void)hipEventSynchronize(ev.back());
+            for (size_t i = 1; i < ev.size(); ++i) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, ev[i - 1], ev[i]);
+                switch (kind[i]) {
+                    case 1: t->fp_ms += ms; t->fp_launches++; break;
+                    case 2: t->bp_ms += ms; t->bp_launches++; break;
+                    case 3: t->dual_ms += ms; t->dual_launches++; break;
+                    case 4: t->primal_ms += ms; t->primal_launches++; break;
+                    default: break;
+                }
+            }
+        }
+    };
+
+    // One ST iteration (one No_PO pass over all frames).
+    int issue_iteration(bool batched, Timer* tm) {
+        const bool ray_bp = p.bp == STT_BP_RAY;
+        auto mark = [&](int k) {
+            if (tm) tm->mark(k);
+        };
+        if (batched) {
+            HIPCHK(launch_pd_dual(fb_dev, 0, F, vg, pc, stream));
+            mark(3);
+            HIPCHK(launch_pd_primal(fb_dev, 0, F, vg, pc, stream));
+            mark(4);
+            for (int k = 0; k < maxP; ++k) {
+                const WorkItem* it = items_batch + batch_off[k];
+                HIPCHK(launch_fp(FP_MODE_SART, it, batch_cnt[k], vg, pc.factor, stream));
+                mark(1);
+                int rc = launch_bp_for(it, batch_cnt[k], ray_bp);
+                if (rc) return rc;
+                mark(2);
+            }
+        } else {
+            int off = 0;
+            for (int f = 0; f < F; ++f) {
+                HIPCHK(launch_pd_dual(fb_dev, f, 1, vg, pc, stream));
+                mark(3);
+                HIPCHK(launch_pd_primal(fb_dev, f, 1, vg, pc, stream));
+                mark(4);
+                for (int k = 0; k < P[f]; ++k) {
+                    const WorkItem* it = items_plain + off + k;
+                    HIPCHK(launch_fp(FP_MODE_SART, it, 1, vg, pc.factor, stream));
+                    mark(1);
+                    int rc = launch_bp_for(it, 1, ray_bp);
+                    if (rc) return rc;
+                    mark(2);
+                }
+                off += P[f];
+            }
+        }
+        return STT_OK;
+    }
+
+    int pd_init() {
+        for (int f = 0; f <= F; ++f) {
+            HIPCHK(hipMemcpyAsync(xbar[f], x[f], V * sizeof(float), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemsetAsync(ytv[f], 0, 3 * V * sizeof(float), stream));
+            HIPCHK(hipMemsetAsync(ytp[f], 0, V * sizeof(float), stream));
+            HIPCHK(hipMemsetAsync(ybc[f], 0, V * sizeof(float), stream));
+            HIPCHK(hipMemsetAsync(xprev[f], 0, V * sizeof(float), stream));
+        }
+        pd_ready = true;
+        return STT_OK;
+    }
+
+    void invalidate_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        graph_batched = -1;
+    }
+};
+
+#define CTX_CHECK(ctx) \
+    if (!(ctx)) return fail(STT_ERR_INVALID, "null context")
+
+namespace {
+
+int valid_geometry(const stt_geometry* g) {
+    if (!g) return fail(STT_ERR_INVALID, "null geometry");
+    for (int c = 0; c < 3; ++c) {
+        if (g->vol[c] < 4) return fail(STT_ERR_INVALID, "volume dims must be >= 4");
+        if (!(g->spacing[c] > 0.0f)) return fail(STT_ERR_INVALID, "voxel spacing must be > 0");
+    }
+    if (g->det[0] < 5 || g->det[1] < 5) return fail(STT_ERR_INVALID, "detector must be at least 5x5");
+    if (!(g->ds > 0) || !(g->sdd > 0) || !(g->sid > 0)) return fail(STT_ERR_INVALID, "ds, sid, sdd must be > 0");
+    const double v = (double)g->vol[0] * g->vol[1] * g->vol[2];
+    if (v + tail_pad(g->vol) >= 2147483647.0) return fail(STT_ERR_INVALID, "volume exceeds 2^31 voxels");
+    return STT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* stt_last_error(void) { return g_last_error.c_str(); }
+
+int stt_version(void) { return 100; }
+
+void stt_params_default(stt_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->sigma = 0.9 / 8.0;  // CP_sigma = .9/(L*CP_tau), L = 8 (ST_Tomography.cpp:99-102)
+    p->tau = 1.0;          // CP_tau (:100)
+    p->lambda_tv = 0.1;    // main.cpp:70
+    p->k_tp = 0.0;         // main.cpp:71
+    p->e_bc = 0.0;         // main.cpp:72
+    p->huber = 0.1;        // main.cpp:73
+    p->alpha = 0.1;        // main.cpp:76
+    p->g_max = 0.0f;
+    p->prior = STT_PRIOR_ATV;
+    p->bp = STT_BP_VOXEL;
+    p->interp = STT_INTERP_CUBIC_CLAMPED;
+    p->roi_jtop = 0;
+    p->fp64_reductions = 1;
+}
+
+void stt_params_reference(stt_params* p) {
+    stt_params_default(p);
+    if (!p) return;
+    const int cut[8] = {8, 8, 8, 45, 8, 8, 30, 20};  // ST_Tomography.cpp:114
+    for (int i = 0; i < 8; ++i) p->roi_cut[i] = cut[i];
+    p->roi_jtop = 7;  // ST_Tomography.cpp:324
+}
+
+int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, const int* views_per_frame,
+                   stt_ctx** out) {
+    if (!out) return fail(STT_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = valid_geometry(g);
+    if (rc) return rc;
+    if (!p) return fail(STT_ERR_INVALID, "null params");
+    if (nframes < 1 || !views_per_frame) return fail(STT_ERR_INVALID, "nframes must be >= 1 with a view count per frame");
+    for (int f = 0; f < nframes; ++f)
+        if (views_per_frame[f] < 1) return fail(STT_ERR_INVALID, "every frame needs at least one view");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(STT_ERR_NODEVICE, "no HIP device visible");
+    auto* c = new stt_ctx();
+    c->g = *g;
+    c->p = *p;
+    c->F = nframes;
+    c->P.assign(views_per_frame, views_per_frame + nframes);
+    c->view_off.assign(nframes + 1, 0);
+    for (int f = 0; f < nframes; ++f) c->view_off[f + 1] = c->view_off[f] + c->P[f];
+    c->total_views = c->view_off[nframes];
+    c->maxP = *std::max_element(c->P.begin(), c->P.end());
+    (void)hipGetDevice(&c->device);
+    auto bail = [&](int code) {
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(STT_ERR_HIP, "hipStreamCreate failed"));
+    c->V = (size_t)g->vol[0] * g->vol[1] * g->vol[2];
+    c->Vpad = c->V + tail_pad(g->vol);
+    c->D = (size_t)g->det[0] * g->det[1];
+    c->vg.nx = g->vol[0];
+    c->vg.ny = g->vol[1];
+    c->vg.nz = g->vol[2];
+    c->vg.W = g->det[0];
+    c->vg.H = g->det[1];
+    c->vg.sd = sample_distance(*g);
+    c->vg.bx = (float)(g->vol[0] - 1);
+    c->vg.by = (float)(g->vol[1] - 1);
+    c->vg.bz = (float)(g->vol[2] - 1);
+    c->vg.max_samples = 2 * (g->vol[0] + g->vol[1] + g->vol[2]) + 16;
+    c->derive_consts();
+    const int F = nframes;
+    c->x.resize(F + 1);
+    c->xbar.resize(F + 1);
+    c->xprev.resize(F + 1);
+    c->ytv.resize(F + 1);
+    c->ytp.resize(F + 1);
+    c->ybc.resize(F + 1);
+    c->flow.resize(F);
+    c->b.resize(F);
+    c->raylen_f.resize(F);
+    c->corr_f.resize(F);
+    for (int f = 0; f <= F; ++f) {
+        if ((rc = c->alloc(&c->x[f], c->Vpad))) return bail(rc);
+        if ((rc = c->alloc(&c->xbar[f], c->V))) return bail(rc);
+        if ((rc = c->alloc(&c->xprev[f], c->V))) return bail(rc);
+        if ((rc = c->alloc(&c->ytv[f], 3 * c->V))) return bail(rc);
+        if ((rc = c->alloc(&c->ytp[f], c->V))) return bail(rc);
+        if ((rc = c->alloc(&c->ybc[f], c->V))) return bail(rc);
+    }
+    for (int f = 0; f < F; ++f) {
+        if ((rc = c->alloc(&c->flow[f], 3 * c->V))) return bail(rc);
+        if ((rc = c->alloc(&c->b[f], c->D * c->P[f]))) return bail(rc);
+        if ((rc = c->alloc(&c->raylen_f[f], c->D))) return bail(rc);
+        if ((rc = c->alloc(&c->corr_f[f], c->D))) return bail(rc);
+    }
+    if (p->bp == STT_BP_RAY) {
+        c->contrib_f.resize(F);
+        c->wsum_f.resize(F);
+        for (int f = 0; f < F; ++f) {
+            if ((rc = c->alloc(&c->contrib_f[f], c->Vpad))) return bail(rc);
+            if ((rc = c->alloc(&c->wsum_f[f], c->Vpad))) return bail(rc);
+        }
+    }
+    if ((rc = c->alloc(&c->raylen_shared, c->D))) return bail(rc);
+    if ((rc = c->alloc(&c->zero_flow, 3 * c->V))) return bail(rc);
+    if ((rc = c->alloc_t(&c->views_dev, (size_t)c->total_views))) return bail(rc);
+    if ((rc = c->alloc_t(&c->fb_dev, (size_t)F + 1))) return bail(rc);
+    if ((rc = c->alloc_t(&c->items_plain, (size_t)c->total_views))) return bail(rc);
+    if ((rc = c->alloc_t(&c->items_batch, (size_t)c->total_views))) return bail(rc);
+    c->scratch_cap = std::max(64, c->maxP + F);
+    if ((rc = c->alloc_t(&c->items_scratch, (size_t)c->scratch_cap))) return bail(rc);
+    c->views_host.resize(c->total_views);
+    c->views_set.assign(F, 0);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(STT_ERR_HIP, "stream sync failed"));
+    *out = c;
+    return STT_OK;
+}
+
+int stt_ctx_destroy(stt_ctx* ctx) {
+    CTX_CHECK(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return STT_OK;
+}
+
+int stt_ctx_set_params(stt_ctx* ctx, const stt_params* p) {
+    CTX_CHECK(ctx);
+    if (!p) return fail(STT_ERR_INVALID, "null params");
+    if (p->bp != ctx->p.bp) return fail(STT_ERR_INVALID, "the backprojector is fixed at context creation");
+    const float old_gmax = ctx->gmax;
+    const bool from_data = ctx->gmax_from_data;
+    ctx->p = *p;
+    ctx->derive_consts();
+    if (p->g_max <= 0 && from_data) {
+        ctx->gmax = old_gmax;  // keep the data-derived value
+        ctx->pc.gmax = old_gmax;
+    }
+    ctx->invalidate_graph();
+    return STT_OK;
+}
+
+int stt_ctx_get_gmax(stt_ctx* ctx, float* g_max) {
+    CTX_CHECK(ctx);
+    if (!g_max) return fail(STT_ERR_INVALID, "null output");
+    *g_max = ctx->gmax;
+    return STT_OK;
+}
+
+int stt_set_view_quaternions(stt_ctx* ctx, int frame, const float* q, int n) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    if (!q || n != ctx->P[frame]) return fail(STT_ERR_INVALID, "view count must equal views_per_frame[frame]");
+    ViewRec* dst = ctx->views_host.data() + ctx->view_off[frame];
+    for (int i = 0; i < n; ++i) dst[i] = build_view(ctx->g, q + 4 * i);
+    HIPCHK(hipMemcpyAsync(ctx->views_dev + ctx->view_off[frame], dst, n * sizeof(ViewRec), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->views_set[frame] = 1;
+    ctx->tables_dirty = true;
+    ctx->invalidate_graph();
+    return STT_OK;
+}
+
+int stt_set_views(stt_ctx* ctx, int frame, const float* angles_deg, int n) {
+    CTX_CHECK(ctx);
+    if (!angles_deg) return fail(STT_ERR_INVALID, "null angles");
+    if (n < 0) return fail(STT_ERR_INVALID, "negative view count");
+    std::vector<float> q(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) quat_about_y(angles_deg[i], &q[4 * i]);
+    return stt_set_view_quaternions(ctx, frame, q.data(), n);
+}
+
+static int buffer_of(stt_ctx* ctx, int which, int frame, float** ptr, size_t* n) {
+    if (frame < 0 || frame > ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    switch (which) {
+        case STT_BUF_XBAR: *ptr = ctx->xbar[frame]; *n = ctx->V; break;
+        case STT_BUF_XPREV: *ptr = ctx->xprev[frame]; *n = ctx->V; break;
+        case STT_BUF_YTV: *ptr = ctx->ytv[frame]; *n = 3 * ctx->V; break;
+        case STT_BUF_YTP: *ptr = ctx->ytp[frame]; *n = ctx->V; break;
+        case STT_BUF_YBC: *ptr = ctx->ybc[frame]; *n = ctx->V; break;
+        default: return fail(STT_ERR_INVALID, "unknown buffer id");
+    }
+    return STT_OK;
+}
+
+int stt_get_buffer(stt_ctx* ctx, int which, int frame, float* out) {
+    CTX_CHECK(ctx);
+    if (!out) return fail(STT_ERR_INVALID, "null output");
+    float* src;
+    size_t n;
+    int rc = buffer_of(ctx, which, frame, &src, &n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, src, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_set_buffer(stt_ctx* ctx, int which, int frame, const float* in) {
+    CTX_CHECK(ctx);
+    if (!in) return fail(STT_ERR_INVALID, "null input");
+    float* dst;
+    size_t n;
+    int rc = buffer_of(ctx, which, frame, &dst, &n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(dst, in, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_set_volume(stt_ctx* ctx, int frame, const float* vol) {
+    CTX_CHECK(ctx);
+    if (!vol) return fail(STT_ERR_INVALID, "null volume");
+    if (frame < 0 || frame > ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    HIPCHK(hipMemcpyAsync(ctx->x[frame], vol, ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_get_volume(stt_ctx* ctx, int frame, float* vol) {
+    CTX_CHECK(ctx);
+    if (!vol) return fail(STT_ERR_INVALID, "null volume");
+    if (frame < 0 || frame > ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    HIPCHK(hipMemcpyAsync(vol, ctx->x[frame], ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_set_volume_dev(stt_ctx* ctx, int frame, const float* vol_dev) {
+    CTX_CHECK(ctx);
+    if (!vol_dev) return fail(STT_ERR_INVALID, "null volume");
+    if (frame < 0 || frame > ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    HIPCHK(hipMemcpyAsync(ctx->x[frame], vol_dev, ctx->V * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_get_volume_dev(stt_ctx* ctx, int frame, float* vol_dev) {
+    CTX_CHECK(ctx);
+    if (!vol_dev) return fail(STT_ERR_INVALID, "null volume");
+    if (frame < 0 || frame > ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    HIPCHK(hipMemcpyAsync(vol_dev, ctx->x[frame], ctx->V * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_set_flow(stt_ctx* ctx, int pair, const float* uvw) {
+    CTX_CHECK(ctx);
+    if (!uvw) return fail(STT_ERR_INVALID, "null flow");
+    if (pair < 0 || pair >= ctx->F) return fail(STT_ERR_INVALID, "flow pair out of range");
+    HIPCHK(hipMemcpyAsync(ctx->flow[pair], uvw, 3 * ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_set_projections(stt_ctx* ctx, int frame, const float* b) {
+    CTX_CHECK(ctx);
+    if (!b) return fail(STT_ERR_INVALID, "null projections");
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    const size_t n = ctx->D * ctx->P[frame];
+    HIPCHK(hipMemcpyAsync(ctx->b[frame], b, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (frame == 0 && ctx->gmax_from_data) {
+        // G_Max = max(frame-0 projections)/4 (ST_Tomography.cpp:356-357)
+        float mx = -3.402823466e38f;
+        for (size_t k = 0; k < n; ++k) mx = std::max(mx, b[k]);
+        ctx->gmax = mx / 4.0f;
+        ctx->pc.gmax = ctx->gmax;
+        ctx->invalidate_graph();
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* raylen, float* weight) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
+        return fail(STT_ERR_INVALID, "frame/view out of range");
+    if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
+    const size_t D = ctx->D;
+    float *dax = nullptr, *drl = nullptr, *dw = nullptr;
+    HIPCHK(hipMallocAsync((void**)&dax, 3 * D * sizeof(float), ctx->stream));
+    drl = dax + D;
+    dw = drl + D;
+    if (raylen) HIPCHK(hipMemcpyAsync(drl, raylen, D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    else HIPCHK(hipMemsetAsync(drl, 0, D * sizeof(float), ctx->stream));
+    WorkItem it = ctx->make_item(frame, view, true);
+    it.ax = dax;
+    it.raylen = drl;
+    it.weight = dw;
+    int rc = ctx->upload_scratch({it});
+    if (rc) return rc;
+    HIPCHK(launch_fp(FP_MODE_FP, ctx->items_scratch, 1, ctx->vg, 1.0f, ctx->stream));
+    if (Ax) HIPCHK(hipMemcpyAsync(Ax, dax, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (raylen) HIPCHK(hipMemcpyAsync(raylen, drl, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (weight) HIPCHK(hipMemcpyAsync(weight, dw, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(dax, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_backproject_voxel(stt_ctx* ctx, int frame, int view, const float* corr, double alpha) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
+        return fail(STT_ERR_INVALID, "frame/view out of range");
+    if (!corr) return fail(STT_ERR_INVALID, "null correction");
+    if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
+    HIPCHK(hipMemcpyAsync(ctx->corr_f[frame], corr, ctx->D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    WorkItem it = ctx->make_item(frame, view, true);
+    it.last = 0;
+    int rc = ctx->upload_scratch({it});
+    if (rc) return rc;
+    PdConst pc = ctx->pc;
+    pc.alpha = (float)alpha;
+    HIPCHK(launch_bp_voxel(ctx->items_scratch, 1, ctx->vg, pc, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr, float* contrib, float* wsum) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
+        return fail(STT_ERR_INVALID, "frame/view out of range");
+    if (!corr) return fail(STT_ERR_INVALID, "null correction");
+    if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
+    float* acc = nullptr;
+    HIPCHK(hipMallocAsync((void**)&acc, 2 * ctx->Vpad * sizeof(float), ctx->stream));
+    HIPCHK(hipMemsetAsync(acc, 0, 2 * ctx->Vpad * sizeof(float), ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->corr_f[frame], corr, ctx->D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    WorkItem it = ctx->make_item(frame, view, true);
+    it.contrib = acc;
+    it.wsum = acc + ctx->Vpad;
+    it.raylen = nullptr;
+    it.last = 0;
+    int rc = ctx->upload_scratch({it});
+    if (rc) return rc;
+    HIPCHK(launch_fp(FP_MODE_SCATTER, ctx->items_scratch, 1, ctx->vg, 1.0f, ctx->stream));
+    if (contrib) HIPCHK(hipMemcpyAsync(contrib, acc, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (wsum) HIPCHK(hipMemcpyAsync(wsum, acc + ctx->Vpad, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(launch_bp_ray_apply(ctx->items_scratch, 1, ctx->vg, ctx->pc, ctx->stream));
+    HIPCHK(hipFreeAsync(acc, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
+        return fail(STT_ERR_INVALID, "frame/view out of range");
+    if (!y || !out) return fail(STT_ERR_INVALID, "null argument");
+    if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
+    float* acc = nullptr;
+    HIPCHK(hipMallocAsync((void**)&acc, (ctx->Vpad + ctx->D) * sizeof(float), ctx->stream));
+    HIPCHK(hipMemsetAsync(acc, 0, ctx->Vpad * sizeof(float), ctx->stream));
+    HIPCHK(hipMemcpyAsync(acc, out, ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    float* dy = acc + ctx->Vpad;
+    HIPCHK(hipMemcpyAsync(dy, y, ctx->D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    WorkItem it = ctx->make_item(frame, view, true);
+    it.corr = dy;
+    it.contrib = acc;
+    it.wsum = nullptr;
+    it.raylen = nullptr;
+    int rc = ctx->upload_scratch({it});
+    if (rc) return rc;
+    // A x = sd * sum_s trilinear(x, p_s), so A^T y scatters y*sd with the trilinear weights.
+    HIPCHK(launch_fp(FP_MODE_SCATTER, ctx->items_scratch, 1, ctx->vg, ctx->vg.sd, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, acc, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(acc, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    if (n < 1 || n > ctx->P[frame]) return fail(STT_ERR_INVALID, "view count out of range");
+    if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
+    std::vector<WorkItem> items;
+    for (int i = 0; i < n; ++i) {
+        const int v = order ? order[i] : i;
+        if (v < 0 || v >= ctx->P[frame]) return fail(STT_ERR_INVALID, "view index out of range in order");
+        WorkItem it = ctx->make_item(frame, v, true);
+        it.last = (i == n - 1) ? 1 : 0;
+        it.xbar = nullptr;  // ROI mask only; the extrapolation belongs to the PD loop
+        items.push_back(it);
+    }
+    const bool ray_bp = ctx->p.bp == STT_BP_RAY;
+    for (int i = 0; i < n; ++i) {
+        int rc = ctx->upload_scratch({items[i]});
+        if (rc) return rc;
+        HIPCHK(launch_fp(FP_MODE_SART, ctx->items_scratch, 1, ctx->vg, ctx->pc.factor, ctx->stream));
+        rc = ctx->launch_bp_for(ctx->items_scratch, 1, ray_bp);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));  // the scratch slot is reused by the next view
+    }
+    if (ray_bp) HIPCHK(hipMemsetAsync(ctx->raylen_shared, 0, ctx->D * sizeof(float), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* out, int interp) {
+    CTX_CHECK(ctx);
+    if (!uvw || !src || !out) return fail(STT_ERR_INVALID, "null argument");
+    if (interp != STT_INTERP_CUBIC_CLAMPED && interp != STT_INTERP_TRILINEAR)
+        return fail(STT_ERR_INVALID, "unknown interpolation");
+    const size_t V = ctx->V;
+    float* buf = nullptr;
+    HIPCHK(hipMallocAsync((void**)&buf, 5 * V * sizeof(float), ctx->stream));
+    HIPCHK(hipMemcpyAsync(buf, uvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(buf + 3 * V, src, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_warp(buf, buf + 3 * V, buf + 4 * V, ctx->vg, dt, interp == STT_INTERP_CUBIC_CLAMPED,
+                       ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, buf + 4 * V, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(buf, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, const float* g, float* gsrc,
+                 float* guvw) {
+    CTX_CHECK(ctx);
+    (void)uvw; (void)src; (void)dt; (void)g; (void)gsrc; (void)guvw;
+    return fail(STT_ERR_STATE, "stt_warp_vjp: not available in this build");
+}
+
+int stt_pd_init(stt_ctx* ctx) {
+    CTX_CHECK(ctx);
+    int rc = ctx->pd_init();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_pd_dual(stt_ctx* ctx, int frame) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    int rc = ctx->build_tables();
+    if (rc) return rc;
+    HIPCHK(launch_pd_dual(ctx->fb_dev, frame, 1, ctx->vg, ctx->pc, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_pd_primal(stt_ctx* ctx, int frame) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    int rc = ctx->build_tables();
+    if (rc) return rc;
+    HIPCHK(launch_pd_primal(ctx->fb_dev, frame, 1, ctx->vg, ctx->pc, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_pd_extrapolate(stt_ctx* ctx, int frame) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    int rc = ctx->build_tables();
+    if (rc) return rc;
+    HIPCHK(launch_extrapolate(ctx->fb_dev, frame, 1, ctx->vg, ctx->pc, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timing* timing) {
+    CTX_CHECK(ctx);
+    if (n_iters < 0) return fail(STT_ERR_INVALID, "negative iteration count");
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    rc = ctx->build_tables();
+    if (rc) return rc;
+    if (!ctx->pd_ready && (rc = ctx->pd_init())) return rc;
+    if (timing) std::memset(timing, 0, sizeof(*timing));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+    }
+    if (use_graph) {
+        if (!ctx->graph_exec || ctx->graph_batched != (batched ? 1 : 0)) {
+            ctx->invalidate_graph();
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            rc = ctx->issue_iteration(batched != 0, nullptr);
+            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ce != hipSuccess) return fail(STT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(STT_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            ctx->graph_batched = batched ? 1 : 0;
+        }
+        for (int it = 0; it < n_iters; ++it) HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    } else {
+        for (int it = 0; it < n_iters; ++it) {
+            stt_ctx::Timer tm(timing, ctx->stream);
+            tm.mark(0);
+            rc = ctx->issue_iteration(batched != 0, timing ? &tm : nullptr);
+            if (rc) return rc;
+            tm.collect();
+        }
+    }
+    if (timing) {
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        timing->total_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return STT_OK;
+}
+
+int stt_synchronize(stt_ctx* ctx) {
+    CTX_CHECK(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_residual_norm(stt_ctx* ctx, double* r2) {
+    CTX_CHECK(ctx);
+    (void)r2;
+    return fail(STT_ERR_STATE, "stt_residual_norm: not available in this build");
+}
+
+int stt_power_iteration(stt_ctx* ctx, int iters, double* L) {
+    CTX_CHECK(ctx);
+    (void)iters; (void)L;
+    return fail(STT_ERR_STATE, "stt_power_iteration: not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// stt_device.hpp — records shared by the host library and the HIP kernels.
+//
+// Everything the kernels need about one projection sits in a 256-byte
+// ViewRec built once per view on the host; a launch covers several frames
+// (the batched form) through a table of WorkItems, one per frame taking part.
+#pragma once
+#include <cstdint>
+
+namespace stt {
+
+// One projection's cone geometry, in the precision the reference uses.
+// Matrices are column-major (Matrix4x4::ele order).
+struct alignas(256) ViewRec {
+    float P2M[16];       // pixel -> model (TypicalRayProducer.cpp:204-205)
+    float MVV2M[16];     // model-view * volume-to-model, for the voxel BP (ST_Tomography.cpp:699)
+    float eye[4];        // EyePos (TypicalRayProducer.core.h:255-258)
+    float o[4];          // voxel_zero = M2V * EyePos
+    float m2v[4];        // diagonal of M2V (1/spacing)
+    float halfW, halfH;  // m_halfFrustumWidth/Height (ST_Tomography.cpp:229-230)
+    float sdd, ds;
+    float pad[12];
+};
+static_assert(sizeof(ViewRec) == 256, "ViewRec must stay one 256-byte record");
+
+// Static per-context constants passed by value to every kernel.
+struct VolGeom {
+    int nx, ny, nz;      // volume dims
+    int W, H;            // detector
+    float sd;            // sample distance ||spacing|| (ST_Tomography.cpp:300)
+    float bx, by, bz;    // CroppingBounds upper faces dims-1 (TypicalRayProducer.core.h:241-244)
+    int max_samples;     // hard cap of the sample loop (never reached by a valid ray)
+};
+
+// Solver constants (reference globals ST_Tomography.cpp:88-111, 219).
+struct PdConst {
+    float sigma, tau, lambda, ktp, ebc, huber_factor, gmax, factor;
+    float alpha;
+    int cut[8];
+    int jtop;
+    int prior;
+    int interp;
+    int nframes;
+};
+
+// One frame's share of a projector / backprojector launch.
+struct WorkItem {
+    const ViewRec* view;
+    float* vol;          // x[f]
+    const float* b;      // measured projection of this view (W*H)
+    float* raylen;       // stale ray-length buffer (read/write)
+    float* corr;         // correction image / ray-BP input (W*H)
+    float* ax;           // forward projection output (stand-alone FP)
+    float* weight;       // sample-count output (stand-alone FP)
+    float* contrib;      // ray-BP accumulators (padded V)
+    float* wsum;
+    float* xprev;        // X_prev[f] (finalize step)
+    float* xbar;         // X_bar[f] (finalize step)
+    int frame;
+    int last;            // 1: last view of the sweep -> x *= imROI and extrapolate
+};
+
+// Per-frame buffer table of the primal-dual kernels (F+1 entries).
+struct FrameBufs {
+    float* x;
+    float* xbar;
+    float* xprev;
+    float* ytv;   // 3 planes
+    float* ytp;
+    float* ybc;
+    float* flow;  // u_f, 3 planes (zero for the last frame)
+    float* pad;
+};
+
+}  // namespace stt

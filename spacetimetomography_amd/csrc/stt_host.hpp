@@ -1,0 +1,16 @@
+// stt_host.hpp — host-side helpers of the library (not part of the C-ABI).
+#pragma once
+#include "../../include/stt.h"
+#include "stt_device.hpp"
+
+namespace stt {
+
+// Quaternion (x, y, z, w) of a rotation by theta degrees about +y, as the
+// reference's InitViews builds it.
+void quat_about_y(float theta_deg, float q[4]);
+// Per-view geometry record (fp32, reference arithmetic order).
+ViewRec build_view(const stt_geometry& g, const float q[4]);
+// Sample distance ||spacing||.
+float sample_distance(const stt_geometry& g);
+
+}  // namespace stt

@@ -1,0 +1,25 @@
+// stt_kernels.hpp — host-side launchers of the HIP kernels in stt_kernels.hip.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include "stt_device.hpp"
+
+namespace stt {
+
+enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2 };
+
+// Projector over nitems work items (one per frame), on stream s.
+//   FP_MODE_FP      : ax, raylen (hit rays), weight
+//   FP_MODE_SART    : corr = (F b - F Ax) / (F len + 1), raylen updated
+//   FP_MODE_SCATTER : contrib += factor*corr*w, wsum += w (wsum may be null)
+hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s);
+hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
+hipError_t launch_bp_ray_apply(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
+hipError_t launch_pd_dual(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s);
+hipError_t launch_pd_primal(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s);
+hipError_t launch_extrapolate(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc,
+                              hipStream_t s);
+hipError_t launch_warp(const float* uvw, const float* src, float* out, const VolGeom& g, float dt, int cubic,
+                       hipStream_t s);
+
+}  // namespace stt

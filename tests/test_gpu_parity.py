@@ -1,0 +1,255 @@
+"""Parity of the HIP path with the CPU oracle (run on an MI355X).
+
+Every test prints the case it runs before touching the device.  Tolerances:
+integer-valued outputs (sample counts) and the ray lengths are bit-exact;
+fp32 line integrals within relative L2 1e-5; volumes after one or more
+sweeps / iterations within relative L2 1e-4 (FMA contraction and float-vs-
+double cubic evaluation are the only sources of difference).
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import spacetimetomography_amd as stt
+from helpers import rel_l2, smooth_flow
+
+pytestmark = pytest.mark.gpu
+
+FP_CASES = [
+    # vol, det, spacing, ds, sid, sdd, oxyz, angle
+    ((24, 24, 24), (40, 36), (1.0, 1.0, 1.0), 1.0, 60.0, 100.0, (0.0, 0.0, 0.0), 0.0),
+    ((24, 20, 28), (48, 40), (1.0, 1.0, 1.0), 0.8, 55.0, 110.0, (1.5, -2.0, 0.5), 37.0),
+    ((20, 26, 22), (36, 44), (0.7, 1.3, 0.9), 0.6, 50.0, 95.0, (0.0, 1.0, -1.0), 215.0),
+    ((32, 32, 32), (30, 30), (1.0, 1.0, 1.0), 1.0, 70.0, 120.0, (0.0, 0.0, 0.0), 90.0),   # partial misses
+    ((17, 19, 23), (41, 29), (1.1, 0.9, 1.0), 1.2, 45.0, 90.0, (-1.0, 0.5, 2.0), 300.0),
+]
+
+
+def _ctx(vol, det, spacing=(1, 1, 1), ds=1.0, sid=60.0, sdd=100.0, oxyz=(0, 0, 0), P=(1,), params=None):
+    g = stt.Geometry.make(vol, det, spacing, ds, sid, sdd, oxyz)
+    p = params if params is not None else stt.params_default()
+    return g, stt.Context(g, p, list(P))
+
+
+@pytest.mark.parametrize("case", range(len(FP_CASES)))
+def test_fp_matches_oracle(case):
+    vol, det, sp, ds, sid, sdd, oxyz, ang = FP_CASES[case]
+    print(f"[fp] case={case} vol={vol} det={det} angle={ang}", flush=True)
+    rng = np.random.default_rng(100 + case)
+    g, ctx = _ctx(vol, det, sp, ds, sid, sdd, oxyz, P=(1,))
+    x = rng.random(int(np.prod(vol)), dtype=np.float32)
+    ctx.set_volume(0, x)
+    ctx.set_views(0, [ang])
+    rl0 = rng.random(det[0] * det[1], dtype=np.float32)  # stale values must survive on missed rays
+    ax, rl, w = ctx.forward_project(0, 0, raylen=rl0)
+    q = O.quat_y(ang)
+    oax, orl, ow = O.fp(g, q, x, raylen=rl0)
+    assert np.array_equal(w, ow), f"sample counts differ at {np.count_nonzero(w != ow)} rays"
+    assert np.array_equal(rl, orl), "ray lengths differ"
+    assert rel_l2(ax, oax) <= 1e-5
+    ax64, _, _ = O.fp(g, q, x.astype(np.float64), f64mode=True)
+    print(f"  relL2 vs fp32 oracle {rel_l2(ax, oax):.2e}, vs fp64 oracle {rel_l2(ax, ax64):.2e}")
+    ctx.close()
+
+
+def test_fp_constant_volume_closed_form():
+    """Constant volume c: every hit ray integrates to c * sd * (number of samples)."""
+    print("[fp-const] 32^3, 48x48", flush=True)
+    g, ctx = _ctx((32, 32, 32), (48, 48), P=(3,))
+    c = 0.75
+    ctx.set_volume(0, np.full(32 ** 3, c, np.float32))
+    ctx.set_views(0, [0.0, 45.0, 123.4])
+    sd = np.float32(np.sqrt(3.0))
+    for v in range(3):
+        ax, rl, w = ctx.forward_project(0, v)
+        hit = w > 0
+        np.testing.assert_allclose(ax[hit], c * sd * w[hit], rtol=2e-6)
+        assert np.all(ax[~hit] == 0)
+    ctx.close()
+
+
+def test_bp_voxel_matches_oracle():
+    print("[bp-voxel] 28x24x30 det 48x44", flush=True)
+    rng = np.random.default_rng(7)
+    vol, det = (28, 24, 30), (48, 44)
+    g, ctx = _ctx(vol, det, ds=0.9, oxyz=(0.5, -1.0, 0.0), P=(2,))
+    x = rng.random(int(np.prod(vol)), dtype=np.float32)
+    ctx.set_views(0, [12.0, 200.0])
+    ctx.set_volume(0, x)
+    gmax = ctx.gmax()
+    for v, ang in enumerate([12.0, 200.0]):
+        corr = rng.standard_normal(det[0] * det[1]).astype(np.float32)
+        ctx.backproject_voxel(0, v, corr, 0.3)
+        x = O.bp_voxel(g, O.quat_y(ang), corr, 0.3, gmax, x)
+    got = ctx.get_volume(0)
+    assert rel_l2(got, x) <= 1e-5, rel_l2(got, x)
+    ctx.close()
+
+
+def test_bp_ray_matches_oracle():
+    print("[bp-ray] 20^3 det 32x32", flush=True)
+    rng = np.random.default_rng(8)
+    vol, det = (20, 20, 20), (32, 32)
+    p = stt.params_default()
+    p.bp = stt.STT_BP_RAY
+    g, ctx = _ctx(vol, det, sid=50.0, sdd=90.0, P=(1,), params=p)
+    ctx.set_views(0, [33.0])
+    corr = rng.standard_normal(det[0] * det[1]).astype(np.float32)
+    c, w = ctx.backproject_ray(0, 0, corr)
+    oc, ow = O.bp_ray_scatter(g, O.quat_y(33.0), corr)
+    assert rel_l2(w, ow) <= 1e-6
+    assert rel_l2(c, oc) <= 1e-5
+    ctx.close()
+
+
+def test_fp_vjp_dot_product():
+    """<A x, y> == <x, A^T y> for the matched ray-driven transpose."""
+    print("[fp-vjp] 24^3 det 40x40", flush=True)
+    rng = np.random.default_rng(9)
+    vol, det = (24, 24, 24), (40, 40)
+    g, ctx = _ctx(vol, det, P=(2,))
+    ctx.set_views(0, [10.0, 250.0])
+    x = rng.standard_normal(int(np.prod(vol))).astype(np.float32)
+    y = rng.standard_normal(det[0] * det[1]).astype(np.float32)
+    ctx.set_volume(0, x)
+    for v in range(2):
+        ax, _, _ = ctx.forward_project(0, v)
+        aty = ctx.fp_vjp(0, v, y)
+        lhs = float(np.dot(ax.astype(np.float64), y.astype(np.float64)))
+        rhs = float(np.dot(x.astype(np.float64), aty.astype(np.float64)))
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs)) + 1e-6, (lhs, rhs)
+    ctx.close()
+
+
+@pytest.mark.parametrize("interp", [stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_INTERP_TRILINEAR])
+@pytest.mark.parametrize("dt", [-1.0, 1.0])
+def test_warp_matches_oracle(interp, dt):
+    print(f"[warp] interp={interp} dt={dt}", flush=True)
+    dims = (24, 20, 28)
+    g, ctx = _ctx(dims, (16, 16), P=(1,))
+    rng = np.random.default_rng(11)
+    src = rng.random(int(np.prod(dims)), dtype=np.float32)
+    uvw = smooth_flow(dims, 3.0, seed=12)
+    out = ctx.warp(uvw, src, dt, interp)
+    ref = O.warp(dims, uvw, src, dt, interp)
+    assert rel_l2(out, ref) <= 1e-5, rel_l2(out, ref)
+    ctx.close()
+
+
+def test_warp_zero_flow_identity():
+    print("[warp-zero]", flush=True)
+    dims = (16, 16, 16)
+    g, ctx = _ctx(dims, (16, 16), P=(1,))
+    src = np.random.default_rng(13).random(16 ** 3, dtype=np.float32)
+    out = ctx.warp(np.zeros(3 * 16 ** 3, np.float32), src, -1.0).reshape(16, 16, 16)
+    s = src.reshape(16, 16, 16)
+    idx = np.clip(np.arange(16), 1, 14)
+    assert np.array_equal(out, s[np.ix_(idx, idx, idx)])
+    ctx.close()
+
+
+def _setup_pair(vol, det, P, angles_fn, params, F=1, seed=21, flows=True, x0=None):
+    g = stt.Geometry.make(vol, det, (1, 1, 1), 1.0, 60.0, 100.0)
+    ctx = stt.Context(g, params, list(P))
+    oc = O.OracleCtx(g, params, list(P))
+    rng = np.random.default_rng(seed)
+    V = int(np.prod(vol))
+    for f in range(F):
+        ang = angles_fn(f)
+        ctx.set_views(f, ang)
+        oc.set_views(f, ang)
+        truth = stt.synth_phantom_frame(vol, f, n_ellipsoids=6, seed=1, deform_seed=2, amp=2.0)
+        b = np.concatenate([O.fp(g, O.quat_y(a), truth)[0] for a in ang])
+        ctx.set_projections(f, b)
+        oc.set_projections(f, b)
+    for f in range(F + 1):
+        x = (x0 if x0 is not None else rng.random(V, dtype=np.float32) * 0.2)
+        ctx.set_volume(f, x)
+        oc.set_volume(f, x)
+    if flows:
+        for f in range(F - 1):
+            u = stt.synth_flow(vol, f, deform_seed=2, amp=2.0)
+            ctx.set_flow(f, u)
+            oc.set_flow(f, u)
+    return g, ctx, oc
+
+
+def test_sart_sweep_matches_oracle():
+    print("[sart] 32^3 det 48x48 16 views", flush=True)
+    p = stt.params_default()
+    p.alpha = 0.3
+    ang = list(np.linspace(0, 360, 16, endpoint=False))
+    g, ctx, oc = _setup_pair((32, 32, 32), (48, 48), (16,), lambda f: ang, p)
+    assert abs(ctx.gmax() - oc.gmax()) <= 1e-6 * abs(oc.gmax())
+    order = np.random.default_rng(3).permutation(16).astype(np.int32)
+    ctx.sart_sweep(0, order)
+    oc.sart(0, order)
+    got, ref = ctx.get_volume(0), oc.get_volume(0)
+    assert rel_l2(got, ref) <= 1e-4, rel_l2(got, ref)
+    ctx.close()
+
+
+@pytest.mark.parametrize("interp", [stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_INTERP_TRILINEAR])
+def test_pd_steps_match_oracle(interp):
+    print(f"[pd-steps] interp={interp}", flush=True)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01
+    p.interp = interp
+    p.roi_cut[:] = [2, 3, 1, 4, 2, 2, 5, 4]
+    p.roi_jtop = 3
+    vol = (24, 22, 26)
+    g, ctx, oc = _setup_pair(vol, (32, 32), (4, 4, 4), lambda f: [0, 90, 180, 270], p, F=3)
+    ctx.pd_init()
+    oc.pd_init()
+    for f in range(3):
+        ctx.pd_dual(f)
+        oc.pd_dual(f)
+    for f in range(3):
+        ctx.pd_primal(f)
+        oc.pd_primal(f)
+        ctx.pd_extrapolate(f)
+        oc.pd_extrapolate(f)
+    for f in range(3):
+        for which in (stt.STT_BUF_YTV, stt.STT_BUF_YTP, stt.STT_BUF_YBC, stt.STT_BUF_XBAR, stt.STT_BUF_XPREV):
+            a, b = ctx.get_buffer(which, f), oc.get_buffer(which, f)
+            assert rel_l2(a, b) <= 1e-5, (f, which, rel_l2(a, b))
+        assert rel_l2(ctx.get_volume(f), oc.get_volume(f)) <= 1e-5
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "graph"])
+def test_st_iteration_matches_oracle(mode):
+    print(f"[st-iter] mode={mode} F=3 28^3 det 40x40", flush=True)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    P = (6, 5, 7)
+    ga = lambda f: list(stt.synth_angles(1, P[f], offset=sum(P[:f])))
+    g, ctx, oc = _setup_pair((28, 28, 28), (40, 40), P, ga, p, F=3)
+    batched = mode != "plain"
+    ctx.iterate(2, batched=batched, use_graph=(mode == "graph"))
+    if batched:
+        oc.per_frame_raylen(True)
+    oc.iterate(2)
+    for f in range(3):
+        got, ref = ctx.get_volume(f), oc.get_volume(f)
+        assert rel_l2(got, ref) <= 1e-4, (f, rel_l2(got, ref))
+        xb, oxb = ctx.get_buffer(stt.STT_BUF_XBAR, f), oc.get_buffer(stt.STT_BUF_XBAR, f)
+        assert rel_l2(xb, oxb) <= 1e-4
+    ctx.close()
+
+
+def test_batched_equals_plain():
+    print("[batched-vs-plain] F=4", flush=True)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    P = (5, 8, 3, 6)
+    vol, det = (24, 24, 24), (36, 36)
+    ga = lambda f: list(stt.synth_angles(1, P[f], offset=sum(P[:f])))
+    outs = []
+    for batched in (False, True):
+        g, ctx, oc = _setup_pair(vol, det, P, ga, p, F=4, seed=5)
+        ctx.iterate(3, batched=batched, use_graph=batched)
+        outs.append([ctx.get_volume(f) for f in range(4)])
+        ctx.close()
+    for f in range(4):
+        assert rel_l2(outs[1][f], outs[0][f]) <= 1e-6
