@@ -961,6 +961,24 @@ int o_ctx_get_buffer(o_ctx* c, int which, int f, float* out) {
     });
     return rc;
 }
+int o_ctx_set_buffer(o_ctx* c, int which, int f, const float* in) {
+    int rc = 0;
+    DISPATCH(c, {
+        if (f < 0 || f > s.F) { rc = 1; break; }
+        auto* dst = &s.xbar[f];
+        switch (which) {
+            case STT_BUF_XBAR: dst = &s.xbar[f]; break;
+            case STT_BUF_XPREV: dst = &s.xprev[f]; break;
+            case STT_BUF_YTV: dst = &s.ytv[f]; break;
+            case STT_BUF_YTP: dst = &s.ytp[f]; break;
+            case STT_BUF_YBC: dst = &s.ybc[f]; break;
+            default: rc = 1;
+        }
+        if (rc) break;
+        for (size_t k = 0; k < dst->size(); ++k) (*dst)[k] = in[k];
+    });
+    return rc;
+}
 float o_ctx_gmax(o_ctx* c) {
     float r = 0;
     DISPATCH(c, { r = float(s.gmax); });

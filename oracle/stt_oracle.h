@@ -80,6 +80,7 @@ int    o_ctx_set_flow(o_ctx* c, int pair, const float* uvw);
 int    o_ctx_set_projections(o_ctx* c, int frame, const float* b);
 /* which: STT_BUF_* */
 int    o_ctx_get_buffer(o_ctx* c, int which, int frame, float* out);
+int    o_ctx_set_buffer(o_ctx* c, int which, int frame, const float* in);
 float  o_ctx_gmax(o_ctx* c);
 /* The shared detector raylen buffer (one TypicalRayProducer for all frames). */
 int    o_ctx_get_raylen(o_ctx* c, float* out);

@@ -45,7 +45,9 @@ _sigs = {
     "o_ctx_set_flow": (C.c_int, [_P, C.c_int, _F]),
     "o_ctx_set_projections": (C.c_int, [_P, C.c_int, _F]),
     "o_ctx_get_buffer": (C.c_int, [_P, C.c_int, C.c_int, _F]),
+    "o_ctx_set_buffer": (C.c_int, [_P, C.c_int, C.c_int, _F]),
     "o_ctx_gmax": (C.c_float, [_P]),
+    "o_ctx_time_sample": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D]),
     "o_ctx_get_raylen": (C.c_int, [_P, _F]),
     "o_ctx_sart": (C.c_int, [_P, C.c_int, _I, C.c_int]),
     "o_ctx_pd_init": (C.c_int, [_P]),
@@ -212,6 +214,9 @@ class OracleCtx:
         out = np.zeros(n, np.float32)
         assert orc.o_ctx_get_buffer(self.h, which, f, f32(out)) == 0
         return out
+
+    def set_buffer(self, which, f, a):
+        assert orc.o_ctx_set_buffer(self.h, which, f, f32(np.ascontiguousarray(a, np.float32))) == 0
 
     def gmax(self):
         return orc.o_ctx_gmax(self.h)
