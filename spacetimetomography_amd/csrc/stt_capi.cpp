@@ -318,7 +318,10 @@ int valid_geometry(const stt_geometry* g) {
     if (g->det[0] < 5 || g->det[1] < 5) return fail(STT_ERR_INVALID, "detector must be at least 5x5");
     if (!(g->ds > 0) || !(g->sdd > 0) || !(g->sid > 0)) return fail(STT_ERR_INVALID, "ds, sid, sdd must be > 0");
     const double v = (double)g->vol[0] * g->vol[1] * g->vol[2];
-    if (v + tail_pad(g->vol) >= 2147483647.0) return fail(STT_ERR_INVALID, "volume exceeds 2^31 voxels");
+    if ((v + tail_pad(g->vol)) * 4.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "a volume must stay below 4 GiB");
+    if ((double)g->det[0] * g->det[1] * 4.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "detector too large");
+    if ((double)g->vol[0] * g->vol[1] * 4.0 >= 16777216.0)
+        return fail(STT_ERR_INVALID, "a volume slice must stay below 4M voxels (24-bit offsets)");
     return STT_OK;
 }
 
@@ -397,6 +400,8 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     c->vg.by = (float)(g->vol[1] - 1);
     c->vg.bz = (float)(g->vol[2] - 1);
     c->vg.max_samples = 2 * (g->vol[0] + g->vol[1] + g->vol[2]) + 16;
+    c->vg.vol_bytes = (unsigned)(c->Vpad * sizeof(float));
+    c->vg.det_bytes = (unsigned)((size_t)g->det[0] * g->det[1] * sizeof(float));
     c->derive_consts();
     const int F = nframes;
     c->x.resize(F + 1);

@@ -11,14 +11,17 @@ namespace stt {
 // One projection's cone geometry, in the precision the reference uses.
 // Matrices are column-major (Matrix4x4::ele order).
 struct alignas(256) ViewRec {
-    float P2M[16];       // pixel -> model (TypicalRayProducer.cpp:204-205)
-    float MVV2M[16];     // model-view * volume-to-model, for the voxel BP (ST_Tomography.cpp:699)
-    float eye[4];        // EyePos (TypicalRayProducer.core.h:255-258)
-    float o[4];          // voxel_zero = M2V * EyePos
-    float m2v[4];        // diagonal of M2V (1/spacing)
-    float halfW, halfH;  // m_halfFrustumWidth/Height (ST_Tomography.cpp:229-230)
-    float sdd, ds;
-    float pad[12];
+    float P2M[16];     // pixel -> model (TypicalRayProducer.cpp:204-205)
+    // Voxel BP: rows 0 and 1 of MV*V2M (ST_Tomography.cpp:699) scaled by sdd/ds,
+    // row 2 unscaled, then halfW/ds and halfH/ds, so a voxel's detector position
+    // is px = bp[12] - (row0.v)/(row2.v), py = bp[13] - (row1.v)/(row2.v)
+    // (ST_Tomography.cpp:700-703).
+    float bp[16];
+    float eye[4];      // EyePos (TypicalRayProducer.core.h:255-258)
+    float o[4];        // voxel_zero = M2V * EyePos
+    float m2v[4];      // diagonal of M2V (1/spacing)
+    float halfW, halfH, sdd, ds;
+    float pad[16];
 };
 static_assert(sizeof(ViewRec) == 256, "ViewRec must stay one 256-byte record");
 
@@ -29,6 +32,8 @@ struct VolGeom {
     float sd;            // sample distance ||spacing|| (ST_Tomography.cpp:300)
     float bx, by, bz;    // CroppingBounds upper faces dims-1 (TypicalRayProducer.core.h:241-244)
     int max_samples;     // hard cap of the sample loop (never reached by a valid ray)
+    unsigned vol_bytes;  // bytes of one padded volume allocation (buffer range)
+    unsigned det_bytes;  // bytes of one detector image
 };
 
 // Solver constants (reference globals ST_Tomography.cpp:88-111, 219).

@@ -175,7 +175,15 @@ ViewRec build_view(const stt_geometry& g, const float q[4]) {
     std::memset(&rec, 0, sizeof(rec));
     std::memcpy(rec.P2M, p2m.e, sizeof(rec.P2M));
     const Mat4 mvv2m = modelview * v2m;
-    std::memcpy(rec.MVV2M, mvv2m.e, sizeof(rec.MVV2M));
+    // voxel-BP projection rows folded with the detector scale (double, then fp32)
+    const double k = (double)sdd / g.ds;
+    for (int c = 0; c < 4; ++c) {
+        rec.bp[c] = (float)(k * mvv2m.at(0, c));
+        rec.bp[4 + c] = (float)(k * mvv2m.at(1, c));
+        rec.bp[8 + c] = mvv2m.at(2, c);
+    }
+    rec.bp[12] = (float)(halfW / g.ds);
+    rec.bp[13] = (float)(halfH / g.ds);
     rec.eye[0] = p2m.e[8] / p2m.e[11];
     rec.eye[1] = p2m.e[9] / p2m.e[11];
     rec.eye[2] = p2m.e[10] / p2m.e[11];

@@ -66,51 +66,81 @@ __device__ __forceinline__ void produce_ray(const ViewRec& v, const VolGeom& g, 
     r.tfar = tf;
 }
 
-// T_VolumeAccess::GetScalarValue trilinear (SingleScalarVolume.core.h:87-146).
-// The volume allocation carries a zero tail of nx*ny + nx + 1 floats, so the
-// +1 taps need no clamp: where the reference clamps a tap its weight is 0.
-__device__ __forceinline__ float sample_tri(const float* __restrict__ vol, const VolGeom& g, float px, float py,
-                                            float pz) {
-    px = fminf(fmaxf(px, 0.0f), g.bx);
-    py = fminf(fmaxf(py, 0.0f), g.by);
-    pz = fminf(fmaxf(pz, 0.0f), g.bz);
-    const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const float fx = px - (float)ix, fy = py - (float)iy, fz = pz - (float)iz;
-    const int sxy = g.nx * g.ny;
-    const float* p = vol + (ix + g.nx * iy + sxy * iz);
-    const float v000 = p[0], v100 = p[1];
-    const float v010 = p[g.nx], v110 = p[g.nx + 1];
-    const float v001 = p[sxy], v101 = p[sxy + 1];
-    const float v011 = p[sxy + g.nx], v111 = p[sxy + g.nx + 1];
-    const float gx = 1.0f - fx;
-    const float a = v000 * gx + v100 * fx;
-    const float b = v010 * gx + v110 * fx;
-    const float c = v001 * gx + v101 * fx;
-    const float d = v011 * gx + v111 * fx;
-    const float gy = 1.0f - fy;
-    const float ab = a * gy + b * fy;
-    const float cd = c * gy + d * fy;
-    return ab * (1.0f - fz) + cd * fz;
+// Global-address-space views of pointers read from work tables, so loads and
+// atomics compile to global_* instead of flat_* instructions.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ gfloat* gptr(float* p) { return (gfloat*)p; }
+__device__ __forceinline__ const gfloat* gptr(const float* p) { return (const gfloat*)p; }
+
+// Raw buffer over one padded volume: 32-bit byte offsets, the +1 row and
+// +1 slice taps through SGPR offsets, hardware range check on num_records.
+struct VolView {
+    __amdgpu_buffer_rsrc_t rsrc;
+    unsigned nx4, sxy4, sxynx4;
+    float bx, by, bz;
+};
+
+// Buffer resource word 3 of the gfx9 family (32-bit data format, raw access).
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ __forceinline__ VolView vol_view(const float* vol, const VolGeom& g) {
+    VolView v;
+    v.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)g.vol_bytes, kBufWord3);
+    v.nx4 = 4u * (unsigned)g.nx;
+    v.sxy4 = 4u * (unsigned)(g.nx * g.ny);
+    v.sxynx4 = v.sxy4 + v.nx4;
+    v.bx = g.bx;
+    v.by = g.by;
+    v.bz = g.bz;
+    return v;
 }
 
-// Scatter of the 8 trilinear weights (SingleScalarVolume.core.h:99-105, 128-143).
-__device__ __forceinline__ void scatter_tri(float* __restrict__ contrib, float* __restrict__ wsum, const VolGeom& g,
-                                            float px, float py, float pz, float val) {
-    px = fminf(fmaxf(px, 0.0f), g.bx);
-    py = fminf(fmaxf(py, 0.0f), g.by);
-    pz = fminf(fmaxf(pz, 0.0f), g.bz);
-    const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const float t0 = px - (float)ix, t1 = py - (float)iy, t2 = pz - (float)iz;
+// T_VolumeAccess::GetScalarValue trilinear (SingleScalarVolume.core.h:87-146):
+// clamp to [0, dims-1], taps at floor and floor+1.  The volume allocation
+// carries a zero tail of nx*ny + nx + 1 floats, so the +1 taps need no clamp:
+// where the reference clamps a tap, its weight is exactly 0.
+__device__ __forceinline__ float sample_tri(const VolView& v, float px, float py, float pz) {
+    px = __builtin_amdgcn_fmed3f(px, 0.0f, v.bx);
+    py = __builtin_amdgcn_fmed3f(py, 0.0f, v.by);
+    pz = __builtin_amdgcn_fmed3f(pz, 0.0f, v.bz);
+    const unsigned ix = (unsigned)px, iy = (unsigned)py, iz = (unsigned)pz;
+    const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py), fz = __builtin_amdgcn_fractf(pz);
+    const unsigned off = (ix << 2) + __umul24(iy, v.nx4) + __umul24(iz, v.sxy4);
+    const float2 r00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, 0, 0));
+    const float2 r10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.nx4, 0));
+    const float2 r01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxy4, 0));
+    const float2 r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxynx4, 0));
+    const float a = fmaf(fx, r00.y - r00.x, r00.x);
+    const float b = fmaf(fx, r10.y - r10.x, r10.x);
+    const float c = fmaf(fx, r01.y - r01.x, r01.x);
+    const float d = fmaf(fx, r11.y - r11.x, r11.x);
+    const float ab = fmaf(fy, b - a, a);
+    const float cd = fmaf(fy, d - c, c);
+    return fmaf(fz, cd - ab, ab);
+}
+
+// Scatter of the 8 trilinear weights (SingleScalarVolume.core.h:99-105, 128-143)
+// with hardware fp32 atomics.  Accumulators carry the same zero tail.
+__device__ __forceinline__ void scatter_tri(float* contrib, float* wsum, const VolGeom& g, float px, float py,
+                                            float pz, float val) {
+    px = __builtin_amdgcn_fmed3f(px, 0.0f, g.bx);
+    py = __builtin_amdgcn_fmed3f(py, 0.0f, g.by);
+    pz = __builtin_amdgcn_fmed3f(pz, 0.0f, g.bz);
+    const unsigned ix = (unsigned)px, iy = (unsigned)py, iz = (unsigned)pz;
+    const float t0 = __builtin_amdgcn_fractf(px), t1 = __builtin_amdgcn_fractf(py), t2 = __builtin_amdgcn_fractf(pz);
     const float m0 = 1.0f - t0, m1 = 1.0f - t1, m2 = 1.0f - t2;
     const float w[8] = {m0 * m1 * m2, t0 * m1 * m2, m0 * t1 * m2, t0 * t1 * m2,
                         m0 * m1 * t2, t0 * m1 * t2, m0 * t1 * t2, t0 * t1 * t2};
-    const int sxy = g.nx * g.ny;
-    const int base = ix + g.nx * iy + sxy * iz;
-    const int off[8] = {0, 1, g.nx, g.nx + 1, sxy, sxy + 1, sxy + g.nx, sxy + g.nx + 1};
+    const unsigned nx = (unsigned)g.nx, sxy = (unsigned)(g.nx * g.ny);
+    const unsigned base = ix + __umul24(iy, nx) + __umul24(iz, sxy);
+    const unsigned off[8] = {0u, 1u, nx, nx + 1u, sxy, sxy + 1u, sxy + nx, sxy + nx + 1u};
+    gfloat* c = gptr(contrib) + base;
 #pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        atomicAdd(contrib + base + off[n], val * w[n]);
-        if (wsum) atomicAdd(wsum + base + off[n], w[n]);
+    for (int n = 0; n < 8; ++n) __builtin_amdgcn_global_atomic_fadd_f32(c + off[n], val * w[n]);
+    if (wsum) {
+        gfloat* s = gptr(wsum) + base;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) __builtin_amdgcn_global_atomic_fadd_f32(s + off[n], w[n]);
     }
 }
 
@@ -120,9 +150,8 @@ __device__ __forceinline__ void scatter_tri(float* __restrict__ contrib, float* 
 // interior boundaries are the planes at voxel coordinate 0.5 crossed in the
 // negative direction, where the grid restarts at ceil((t_b - tnear)/sd).
 template <bool SCATTER>
-__device__ __forceinline__ float cast_ray(const float* __restrict__ vol, float* __restrict__ contrib,
-                                          float* __restrict__ wsum, float sval, const VolGeom& g, const Ray& r,
-                                          int& nsamp) {
+__device__ __forceinline__ float cast_ray(const VolView& vol, float* contrib, float* wsum, float sval,
+                                          const VolGeom& g, const Ray& r, int& nsamp) {
     const float sd = g.sd;
     const int dimv[3] = {g.nx, g.ny, g.nz};
     float pos1[3];
@@ -137,7 +166,7 @@ __device__ __forceinline__ float cast_ray(const float* __restrict__ vol, float* 
     float density = 0.0f;
     int n = 1;
     if (SCATTER) scatter_tri(contrib, wsum, g, pos1[0], pos1[1], pos1[2], sval);
-    else density = sample_tri(vol, g, pos1[0], pos1[1], pos1[2]) * sd;
+    else density = sample_tri(vol, pos1[0], pos1[1], pos1[2]) * sd;
     float tMax[3], tDelta[3];
     {
 #pragma clang fp contract(off)
@@ -172,7 +201,7 @@ __device__ __forceinline__ float cast_ray(const float* __restrict__ vol, float* 
         }
         while (t < tb && n < g.max_samples) {
             if (SCATTER) scatter_tri(contrib, wsum, g, px, py, pz, sval);
-            else density = fmaf(sample_tri(vol, g, px, py, pz), sd, density);
+            else density = fmaf(sample_tri(vol, px, py, pz), sd, density);
             ++n;
             t += sd;
             px += step[0];
@@ -222,38 +251,35 @@ __global__ __launch_bounds__(TX* TY) void k_fp(const WorkItem* __restrict__ item
         // Ray BP / adjoint: the BP-mode pass of GenerateImage_MT.
         if (hit) {
             int ns;
-            const float c = it.corr[k] * factor;
-            cast_ray<true>(nullptr, it.contrib, it.wsum, c, g, r, ns);
-            if (it.raylen) it.raylen[k] = r.tfar - r.tnear;
+            const float c = gptr(it.corr)[k] * factor;
+            VolView none{};
+            cast_ray<true>(none, it.contrib, it.wsum, c, g, r, ns);
+            if (it.raylen) gptr(it.raylen)[k] = r.tfar - r.tnear;
         }
         return;
     }
     float value = 0.0f;
     int ns = 0;
-    if (hit) value = cast_ray<false>(it.vol, nullptr, nullptr, 0.0f, g, r, ns);
+    if (hit) value = cast_ray<false>(vol_view(it.vol, g), nullptr, nullptr, 0.0f, g, r, ns);
     if (MODE == FP_MODE_FP) {
-        if (hit && it.raylen) it.raylen[k] = r.tfar - r.tnear;
-        if (it.ax) it.ax[k] = value;
-        if (it.weight) it.weight[k] = (float)ns;
+        if (hit && it.raylen) gptr(it.raylen)[k] = r.tfar - r.tnear;
+        if (it.ax) gptr(it.ax)[k] = value;
+        if (it.weight) gptr(it.weight)[k] = (float)ns;
     } else {
         // SART correction (ST_Tomography.cpp:667-676); pY is zero when read.
 #pragma clang fp contract(off)
         float len;
+        gfloat* rl = gptr(it.raylen);
         if (hit) {
             len = r.tfar - r.tnear;
-            it.raylen[k] = len;
+            rl[k] = len;
         } else {
-            len = it.raylen[k];  // stale value of the last view whose ray k hit
+            len = rl[k];  // stale value of the last view whose ray k hit
         }
-        const float num = factor * it.b[k] - factor * value;
+        const float num = factor * gptr(it.b)[k] - factor * value;
         const float den = factor * len + 1.0f;
-        it.corr[k] = num / den;
+        gptr(it.corr)[k] = num / den;
     }
-}
-
-// Catmull-Rom of cubicInterpolate (ST_Tomography.cpp:183-186).
-__device__ __forceinline__ float cubic_cr(float p0, float p1, float p2, float p3, float x) {
-    return p1 + 0.5f * x * (p2 - p0 + x * (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3 + x * (3.0f * (p1 - p2) + p3 - p0)));
 }
 
 // imROI (ST_Tomography.cpp:303-329).
@@ -277,12 +303,25 @@ __device__ __forceinline__ bool cut_out(const VolGeom& g, const PdConst& pc, int
 __device__ __forceinline__ float finalize_voxel(const WorkItem& it, const VolGeom& g, const PdConst& pc, int x, int y,
                                                 int z, size_t q, float xv) {
     xv *= roi_in(g, pc, x, y, z) ? 1.0f : 0.0f;
-    if (it.xbar && !cut_out(g, pc, x, y, z)) it.xbar[q] = 2.0f * xv - it.xprev[q];
+    if (it.xbar && !cut_out(g, pc, x, y, z)) gptr(it.xbar)[q] = 2.0f * xv - gptr(it.xprev)[q];
     return xv;
 }
 
-// Voxel-driven backprojection (ST_Tomography.cpp:687-731).  64x4 voxels of one
-// z-slice per block; grid.z = z + nz*item.
+// Catmull-Rom weights of cubicInterpolate (ST_Tomography.cpp:183-186),
+// identical polynomial to CImg's cubic (CImg.h:13429-13520).
+__device__ __forceinline__ void cr_weights(float t, float w[4]) {
+    const float t2 = t * t, t3 = t2 * t;
+    w[0] = 0.5f * (-t + 2.0f * t2 - t3);
+    w[1] = 1.0f + 0.5f * (-5.0f * t2 + 3.0f * t3);
+    w[2] = 0.5f * (t + 4.0f * t2 - 3.0f * t3);
+    w[3] = 0.5f * (-t2 + t3);
+}
+
+// Voxel-driven backprojection (ST_Tomography.cpp:687-731): project the voxel
+// centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
+// clamp [0, G_Max].  Atomic-free: one thread per voxel, 64x4 voxels of one
+// z-slice per block (x fastest, coalesced RMW of the volume); grid.z = z + nz*item.
+// The 4x4 detector window is read as four 16-byte rows through a raw buffer.
 __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ items, VolGeom g, PdConst pc) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -292,35 +331,42 @@ __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ i
     const ViewRec& v = *it.view;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
     const float fx = (float)x, fy = (float)y, fz = (float)z;
-    const float* M = v.MVV2M;
-    const float e0 = M[0] * fx + M[4] * fy + M[8] * fz + M[12];
-    const float e1 = M[1] * fx + M[5] * fy + M[9] * fz + M[13];
-    const float e2 = M[2] * fx + M[6] * fy + M[10] * fz + M[14];
-    float pxf = v.halfW - (e0 / e2) * v.sdd;
-    float pyf = v.halfH - (e1 / e2) * v.sdd;
-    pxf = pxf / v.ds;
-    pyf = pyf / v.ds;
+    const float* A = v.bp;
+    const float ex = fmaf(A[0], fx, fmaf(A[1], fy, fmaf(A[2], fz, A[3])));
+    const float ey = fmaf(A[4], fx, fmaf(A[5], fy, fmaf(A[6], fz, A[7])));
+    const float ez = fmaf(A[8], fx, fmaf(A[9], fy, fmaf(A[10], fz, A[11])));
+    const float r = __builtin_amdgcn_rcpf(ez);
+    const float pxf = fmaf(-ex, r, A[12]);
+    const float pyf = fmaf(-ey, r, A[13]);
     const int px = (int)pxf, py = (int)pyf;
     const bool inside = !(px < 2 || px >= g.W - 2 || py < 2 || py >= g.H - 2);
     if (!inside && !it.last) return;
-    float xv = it.vol[q];
+    gfloat* vol = gptr(it.vol);
+    float xv = vol[q];
     if (inside) {
         const float dx = pxf - (float)px, dy = pyf - (float)py;
-        const float* C = it.corr + (size_t)(py - 1) * g.W + (px - 1);
-        float col[4][4];
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)it.corr, (short)0, (int)g.det_bytes, kBufWord3);
+        const unsigned off = 4u * (unsigned)((py - 1) * g.W + (px - 1));
+        const unsigned W4 = 4u * (unsigned)g.W;
+        float wy[4], wx[4];
+        cr_weights(dy, wy);
+        cr_weights(dx, wx);
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int a = 0; a < 4; ++a) col[a][b] = C[b * g.W + a];
-        float arr[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) arr[a] = cubic_cr(col[a][0], col[a][1], col[a][2], col[a][3], dy);
-        const float gv = cubic_cr(arr[0], arr[1], arr[2], arr[3], dx);
-        xv = xv + pc.alpha * gv;
-        xv = tmax(tmin(pc.gmax, xv), 0.0f);
+        for (int b = 0; b < 4; ++b) {
+            const float4 row = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, b * W4, 0));
+            acc.x = fmaf(wy[b], row.x, acc.x);
+            acc.y = fmaf(wy[b], row.y, acc.y);
+            acc.z = fmaf(wy[b], row.z, acc.z);
+            acc.w = fmaf(wy[b], row.w, acc.w);
+        }
+        const float gv = wx[0] * acc.x + wx[1] * acc.y + wx[2] * acc.z + wx[3] * acc.w;
+        xv = fmaf(pc.alpha, gv, xv);
+        xv = __builtin_amdgcn_fmed3f(xv, 0.0f, pc.gmax);
     }
     if (it.last) xv = finalize_voxel(it, g, pc, x, y, z, q, xv);
-    it.vol[q] = xv;
+    vol[q] = xv;
 }
 
 // Ray-BP normalisation (ST_Tomography.cpp:746-762): x += contrib/wsum with the
@@ -332,26 +378,20 @@ __global__ __launch_bounds__(256) void k_bp_ray_apply(const WorkItem* __restrict
     const WorkItem& it = items[blockIdx.z / g.nz];
     if (x >= g.nx || y >= g.ny) return;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    const float m = (float)((double)it.contrib[q] / (double)it.wsum[q]);
-    float xv = it.vol[q] + m;
-    xv = tmax(tmin(pc.gmax, xv), 0.0f);
-    it.contrib[q] = 0.0f;
-    it.wsum[q] = 0.0f;
+    gfloat* contrib = gptr(it.contrib);
+    gfloat* wsum = gptr(it.wsum);
+    gfloat* vol = gptr(it.vol);
+    const float m = (float)((double)contrib[q] / (double)wsum[q]);
+    float xv = vol[q] + m;
+    xv = tmax(tmin(pc.gmax, xv), 0.0f);  // ternaries: NaN from 0/0 becomes 0 as in the reference
+    contrib[q] = 0.0f;
+    wsum[q] = 0.0f;
     if (it.last) xv = finalize_voxel(it, g, pc, x, y, z, q, xv);
-    it.vol[q] = xv;
+    vol[q] = xv;
 }
 
 // ---------------------------------------------------------------------------
 // Warp (warping.h:10-145) on the grid aabb = [0, dims].
-
-// CImg _cubic_atXYZ weights for one axis (CImg.h:13429-13520) in weight form.
-__device__ __forceinline__ void cr_weights(float t, float w[4]) {
-    const float t2 = t * t, t3 = t2 * t;
-    w[0] = 0.5f * (-t + 2.0f * t2 - t3);
-    w[1] = 1.0f + 0.5f * (-5.0f * t2 + 3.0f * t3);
-    w[2] = 0.5f * (t + 4.0f * t2 - 3.0f * t3);
-    w[3] = 0.5f * (-t2 + t3);
-}
 
 struct SamplePt {
     int ix, iy, iz;         // base cell of the clamped grid point
@@ -393,10 +433,10 @@ __device__ __forceinline__ void sample_point(const VolGeom& g, float px, float p
 
 // sample_image (warping.h:49-88): clamped Catmull-Rom, bounded by the min/max
 // of the 8 corners of the cell; or trilinear at the same point.
-__device__ __forceinline__ float sample_img(const float* __restrict__ img, const VolGeom& g, const SamplePt& s,
+__device__ __forceinline__ float sample_img(const gfloat* __restrict__ img, const VolGeom& g, const SamplePt& s,
                                             bool cubic) {
     const int sy = g.nx, sz = g.nx * g.ny;
-    const float* c0 = img + (s.ix + sy * s.iy + sz * s.iz);
+    const gfloat* c0 = img + (s.ix + sy * s.iy + sz * s.iz);
     const float a000 = c0[0], a100 = c0[1], a010 = c0[sy], a110 = c0[sy + 1];
     const float a001 = c0[sz], a101 = c0[sz + 1], a011 = c0[sz + sy], a111 = c0[sz + sy + 1];
     if (!cubic) {
@@ -414,7 +454,7 @@ __device__ __forceinline__ float sample_img(const float* __restrict__ img, const
         float acc_y = 0.0f;
 #pragma unroll
         for (int ky = 0; ky < 4; ++ky) {
-            const float* row = img + (size_t)sy * s.ys[ky] + (size_t)sz * s.zs[kz];
+            const gfloat* row = img + (size_t)sy * s.ys[ky] + (size_t)sz * s.zs[kz];
             const float r = s.wx[0] * row[s.xs[0]] + s.wx[1] * row[s.xs[1]] + s.wx[2] * row[s.xs[2]] +
                             s.wx[3] * row[s.xs[3]];
             acc_y = fmaf(s.wy[ky], r, acc_y);
@@ -428,7 +468,7 @@ __device__ __forceinline__ float sample_img(const float* __restrict__ img, const
 // tdt (= -dt of warp), then the image sample.  The first velocity sample sits
 // exactly on grid node clamp(i,1,n-2), where the clamped cubic returns the node
 // value, so it is read directly.
-__device__ __forceinline__ float warp_voxel(const float* __restrict__ img, const float* __restrict__ uvw,
+__device__ __forceinline__ float warp_voxel(const gfloat* __restrict__ img, const gfloat* __restrict__ uvw,
                                             const VolGeom& g, int i, int j, int k, float tdt, bool cubic) {
     const size_t V = (size_t)g.nx * g.ny * g.nz;
     const int ci = min(max(i, 1), g.nx - 2), cj = min(max(j, 1), g.ny - 2), ck = min(max(k, 1), g.nz - 2);
@@ -460,7 +500,7 @@ __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, con
     const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    out[q] = warp_voxel(src, uvw, g, x, y, z, tdt, cubic != 0);
+    out[q] = warp_voxel(gptr(src), gptr(uvw), g, x, y, z, tdt, cubic != 0);
 }
 
 // Dual step of frame f (ST_Tomography.cpp:502-534); grid.z = z + nz*(f - f0).
@@ -475,18 +515,21 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
     const size_t q = (size_t)x + sy * y + sz * z;
     const FrameBufs& B = fb[f];
     const FrameBufs& B1 = fb[f + 1];
-    const float* xb0 = B.xbar;
-    const float* xb1 = B1.xbar;
+    const gfloat* xb0 = gptr(B.xbar);
+    const gfloat* xb1 = gptr(B1.xbar);
+    gfloat* Ytv = gptr(B.ytv);
+    gfloat* Ytp = gptr(B.ytp);
+    gfloat* Ybc = gptr(B.ybc);
     // m_backward_warp = warp(u_f, X_bar[f+1], dt = -1): trace step +1 (:503)
-    const float bw = warp_voxel(xb1, B.flow, g, x, y, z, 1.0f, pc.interp == 0);
+    const float bw = warp_voxel(xb1, gptr(B.flow), g, x, y, z, 1.0f, pc.interp == 0);
     const float xq = xb0[q];
-    float ytp = B.ytp[q] + pc.sigma * (xb1[q] - xq);  // (:504)
-    float ybc = B.ybc[q] + pc.sigma * (bw - xq);      // (:505)
+    float ytp = Ytp[q] + pc.sigma * (xb1[q] - xq);  // (:504)
+    float ybc = Ybc[q] + pc.sigma * (bw - xq);      // (:505)
     if (x >= 1 && x < g.nx - 1 && y >= 1 && y < g.ny - 1 && z >= 1 && z < g.nz - 1 && !cut_out(g, pc, x, y, z)) {
         const float h = pc.huber_factor;
-        float y0 = B.ytv[q] + pc.sigma * (xb0[q + 1] - xq) * h;
-        float y1 = B.ytv[V + q] + pc.sigma * (xb0[q + sy] - xq) * h;
-        float y2 = B.ytv[2 * V + q] + pc.sigma * (xb0[q + sz] - xq) * h;
+        float y0 = Ytv[q] + pc.sigma * (xb0[q + 1] - xq) * h;
+        float y1 = Ytv[V + q] + pc.sigma * (xb0[q + sy] - xq) * h;
+        float y2 = Ytv[2 * V + q] + pc.sigma * (xb0[q + sz] - xq) * h;
         if (pc.prior == 0) {  // ATV per-component clamp (:523-525)
             y0 = tmax(-pc.lambda, tmin(pc.lambda, y0));
             y1 = tmax(-pc.lambda, tmin(pc.lambda, y1));
@@ -496,14 +539,14 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
             const float sc = nrm > pc.lambda ? pc.lambda / nrm : 1.0f;
             y0 *= sc; y1 *= sc; y2 *= sc;
         }
-        B.ytv[q] = y0;
-        B.ytv[V + q] = y1;
-        B.ytv[2 * V + q] = y2;
+        Ytv[q] = y0;
+        Ytv[V + q] = y1;
+        Ytv[2 * V + q] = y2;
         ytp = ytp * pc.ktp / (pc.ktp + 2.0f * pc.sigma);  // (:528)
         ybc = tmax(-pc.ebc, tmin(pc.ebc, ybc));           // (:530)
     }
-    B.ytp[q] = ytp;
-    B.ybc[q] = ybc;
+    Ytp[q] = ytp;
+    Ybc[q] = ybc;
 }
 
 // Primal step of frame f (ST_Tomography.cpp:536-578).
@@ -517,24 +560,27 @@ __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__
     const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
     const size_t q = (size_t)x + sy * y + sz * z;
     const FrameBufs& B = fb[f];
-    float xv = B.x[q];
-    B.xprev[q] = xv;  // (:536)
-    if (f == 0) {     // (:540-545)
-        xv = xv - pc.tau * (-B.ytp[q]);
-        xv = xv - pc.tau * (-B.ybc[q]);
-    } else {          // (:546-555): forward warp of Y_bc[f-1] with u_{f-1}, trace step -1
+    gfloat* X = gptr(B.x);
+    const gfloat* Ytp = gptr(B.ytp);
+    const gfloat* Ybc = gptr(B.ybc);
+    float xv = X[q];
+    gptr(B.xprev)[q] = xv;  // (:536)
+    if (f == 0) {           // (:540-545)
+        xv = xv - pc.tau * (-Ytp[q]);
+        xv = xv - pc.tau * (-Ybc[q]);
+    } else {                // (:546-555): forward warp of Y_bc[f-1] with u_{f-1}, trace step -1
         const FrameBufs& Bm = fb[f - 1];
-        const float fw = warp_voxel(Bm.ybc, Bm.flow, g, x, y, z, -1.0f, pc.interp == 0);
-        xv = xv - pc.tau * (Bm.ytp[q] - B.ytp[q]);
-        xv = xv - pc.tau * (fw - B.ybc[q]);
+        const float fw = warp_voxel(gptr(Bm.ybc), gptr(Bm.flow), g, x, y, z, -1.0f, pc.interp == 0);
+        xv = xv - pc.tau * (gptr(Bm.ytp)[q] - Ytp[q]);
+        xv = xv - pc.tau * (fw - Ybc[q]);
     }
     if (x >= 1 && x < g.nx - 1 && y >= 1 && y < g.ny - 1 && z >= 1 && z < g.nz - 1 && !cut_out(g, pc, x, y, z)) {
-        const float* t = B.ytv;
+        const gfloat* t = gptr(B.ytv);
         const float y1 = -(t[q] - t[q - 1] + t[V + q] - t[V + q - sy] + t[2 * V + q] - t[2 * V + q - sz]);  // (:570-571)
         xv = xv - pc.tau * y1;
         xv = tmax(tmin(pc.gmax, xv), 0.0f);  // (:575)
     }
-    B.x[q] = xv;
+    X[q] = xv;
 }
 
 // Extrapolation X_bar = 2x - X_prev on the ROI (ST_Tomography.cpp:586-604),
@@ -547,7 +593,7 @@ __global__ __launch_bounds__(256) void k_extrapolate(const FrameBufs* __restrict
     if (x >= g.nx || y >= g.ny || cut_out(g, pc, x, y, z)) return;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
     const FrameBufs& B = fb[f];
-    B.xbar[q] = 2.0f * B.x[q] - B.xprev[q];
+    gptr(B.xbar)[q] = 2.0f * gptr(B.x)[q] - gptr(B.xprev)[q];
 }
 
 // ---------------------------------------------------------------------------
