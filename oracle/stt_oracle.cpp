@@ -567,6 +567,7 @@ template <class R> struct OState : o_ctx_base {
     bool per_frame_raylen = false;
     R gmax = R(10);
     bool gmax_set = false;
+    bool pd_ready = false;  // X_bar initialised from x (first PD pass)
     // effective parameters (float globals in the reference)
     R sigma, tau, lam, ktp, ebc, hub, factor;
     double alpha;
@@ -741,6 +742,19 @@ template <class R> static void pd_extrapolate(OState<R>& s, int f) {
                 size_t q = (size_t)xx + (size_t)D[0] * ((size_t)y + (size_t)D[1] * z);
                 xb[q] = R(2) * x[q] - xp[q];  // (:598)
             }
+}
+
+template <class R> static void pd_init_state(OState<R>& s) {
+    // first pass of the PD loop: X_bar <- x for every frame (ST_Tomography.cpp:494-500);
+    // duals and X_prev start at zero (SetData, :339-350)
+    for (int f = 0; f <= s.F; ++f) {
+        s.xbar[f] = s.x[f];
+        std::fill(s.ytv[f].begin(), s.ytv[f].end(), 0);
+        std::fill(s.ytp[f].begin(), s.ytp[f].end(), 0);
+        std::fill(s.ybc[f].begin(), s.ybc[f].end(), 0);
+        std::fill(s.xprev[f].begin(), s.xprev[f].end(), 0);
+    }
+    s.pd_ready = true;
 }
 
 template <class R> static o_ctx_base* make_state(const stt_geometry* g, const stt_params* p, int F, const int* P) {
@@ -1002,15 +1016,7 @@ int o_ctx_sart(o_ctx* c, int f, const int* order, int n) {
     return rc;
 }
 int o_ctx_pd_init(o_ctx* c) {
-    DISPATCH(c, {
-        for (int f = 0; f <= s.F; ++f) {
-            s.xbar[f] = s.x[f];
-            std::fill(s.ytv[f].begin(), s.ytv[f].end(), 0);
-            std::fill(s.ytp[f].begin(), s.ytp[f].end(), 0);
-            std::fill(s.ybc[f].begin(), s.ybc[f].end(), 0);
-            std::fill(s.xprev[f].begin(), s.xprev[f].end(), 0);
-        }
-    });
+    DISPATCH(c, { pd_init_state(s); });
     return 0;
 }
 int o_ctx_pd_dual(o_ctx* c, int f) {
@@ -1033,6 +1039,7 @@ int o_ctx_iterate(o_ctx* c, int n, int faithful) {
             if ((int)s.views[f].size() != s.P[f]) rc = 1;
         if (rc) break;
         const int wt = faithful ? 4 : 0;
+        if (!s.pd_ready) pd_init_state(s);
         for (int it = 0; it < n; ++it)
             for (int f = 0; f < s.F; ++f) {
                 pd_dual(s, f, wt);

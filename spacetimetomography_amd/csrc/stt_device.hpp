@@ -12,16 +12,19 @@ namespace stt {
 // Matrices are column-major (Matrix4x4::ele order).
 struct alignas(256) ViewRec {
     float P2M[16];     // pixel -> model (TypicalRayProducer.cpp:204-205)
-    // Voxel BP: rows 0 and 1 of MV*V2M (ST_Tomography.cpp:699) scaled by sdd/ds,
-    // row 2 unscaled, then halfW/ds and halfH/ds, so a voxel's detector position
-    // is px = bp[12] - (row0.v)/(row2.v), py = bp[13] - (row1.v)/(row2.v)
-    // (ST_Tomography.cpp:700-703).
+    // Voxel BP fast path: rows 0 and 1 of MV*V2M (ST_Tomography.cpp:699) scaled
+    // by sdd/ds, row 2 unscaled, then halfW/ds and halfH/ds, so a voxel's
+    // detector position is px = bp[12] - (row0.v)/(row2.v) (and py alike).
     float bp[16];
     float eye[4];      // EyePos (TypicalRayProducer.core.h:255-258)
     float o[4];        // voxel_zero = M2V * EyePos
     float m2v[4];      // diagonal of M2V (1/spacing)
-    float halfW, halfH, sdd, ds;
-    float pad[16];
+    // Voxel BP exact path (detector-edge decisions): rows 0..2 of MV*V2M as the
+    // reference multiplies them, float m_sdd, double m_halfFrustumWidth/Height
+    // and m_ds (Scene.h:111-116).
+    float mvv2m[12];
+    float sdd, pad0;
+    double halfW, halfH, ds;
 };
 static_assert(sizeof(ViewRec) == 256, "ViewRec must stay one 256-byte record");
 

@@ -184,6 +184,8 @@ ViewRec build_view(const stt_geometry& g, const float q[4]) {
     }
     rec.bp[12] = (float)(halfW / g.ds);
     rec.bp[13] = (float)(halfH / g.ds);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) rec.mvv2m[4 * r + c] = mvv2m.at(r, c);
     rec.eye[0] = p2m.e[8] / p2m.e[11];
     rec.eye[1] = p2m.e[9] / p2m.e[11];
     rec.eye[2] = p2m.e[10] / p2m.e[11];
@@ -191,10 +193,10 @@ ViewRec build_view(const stt_geometry& g, const float q[4]) {
     apply(m2v, rec.eye, rec.o);
     for (int c = 0; c < 3; ++c) rec.m2v[c] = m2v.e[5 * c];
     rec.m2v[3] = 1.0f;
-    rec.halfW = (float)halfW;
-    rec.halfH = (float)halfH;
+    rec.halfW = halfW;
+    rec.halfH = halfH;
     rec.sdd = sdd;
-    rec.ds = (float)g.ds;
+    rec.ds = g.ds;
     return rec;
 }
 

@@ -317,6 +317,24 @@ __device__ __forceinline__ void cr_weights(float t, float w[4]) {
     w[3] = 0.5f * (-t2 + t3);
 }
 
+// Detector position of a voxel centre with the reference's arithmetic
+// (ST_Tomography.cpp:698-703): float matrix-vector product without
+// contraction, IEEE division, m_halfFrustumWidth/m_ds in double.
+__device__ __noinline__ void exact_detector_position(const ViewRec& v, float fx, float fy, float fz, float& pxf,
+                                                     float& pyf) {
+#pragma clang fp contract(off)
+    const float* M = v.mvv2m;
+    const float e0 = ((M[0] * fx + M[1] * fy) + M[2] * fz) + M[3];
+    const float e1 = ((M[4] * fx + M[5] * fy) + M[6] * fz) + M[7];
+    const float e2 = ((M[8] * fx + M[9] * fy) + M[10] * fz) + M[11];
+    const float tx = (e0 / e2) * v.sdd;
+    const float ty = (e1 / e2) * v.sdd;
+    const float ux = (float)(v.halfW - (double)tx);
+    const float uy = (float)(v.halfH - (double)ty);
+    pxf = (float)((double)ux / v.ds);
+    pyf = (float)((double)uy / v.ds);
+}
+
 // Voxel-driven backprojection (ST_Tomography.cpp:687-731): project the voxel
 // centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
 // clamp [0, G_Max].  Atomic-free: one thread per voxel, 64x4 voxels of one
@@ -336,8 +354,13 @@ __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ i
     const float ey = fmaf(A[4], fx, fmaf(A[5], fy, fmaf(A[6], fz, A[7])));
     const float ez = fmaf(A[8], fx, fmaf(A[9], fy, fmaf(A[10], fz, A[11])));
     const float r = __builtin_amdgcn_rcpf(ez);
-    const float pxf = fmaf(-ex, r, A[12]);
-    const float pyf = fmaf(-ey, r, A[13]);
+    float pxf = fmaf(-ex, r, A[12]);
+    float pyf = fmaf(-ey, r, A[13]);
+    // Within 0.01 px of the detector window's edges the update/skip decision is
+    // taken with the reference's own arithmetic (rare, divergent).
+    const float lo = 2.0f, hx = (float)(g.W - 2), hy = (float)(g.H - 2);
+    if (fabsf(pxf - lo) < 0.01f || fabsf(pxf - hx) < 0.01f || fabsf(pyf - lo) < 0.01f || fabsf(pyf - hy) < 0.01f)
+        exact_detector_position(v, fx, fy, fz, pxf, pyf);
     const int px = (int)pxf, py = (int)pyf;
     const bool inside = !(px < 2 || px >= g.W - 2 || py < 2 || py >= g.H - 2);
     if (!inside && !it.last) return;
@@ -393,71 +416,89 @@ __global__ __launch_bounds__(256) void k_bp_ray_apply(const WorkItem* __restrict
 // ---------------------------------------------------------------------------
 // Warp (warping.h:10-145) on the grid aabb = [0, dims].
 
+// Grid point of a warp sample: world_to_cell_grid for aabb = [0, dims]
+// (warping.h:31-36), clamped to [1, dim-2] (warping.h:60-62).  Holds the
+// base cell, Catmull-Rom weights per axis (CImg _cubic_atXYZ neighbours
+// p = x-1, c = x, n = x+1, a = min(x+2, dim-1); where CImg takes n = x because
+// the fraction is 0 the weight of n is 0, so x+1 is read instead) and the byte
+// offsets of the 16 rows of the 4x4x4 neighbourhood.
 struct SamplePt {
-    int ix, iy, iz;         // base cell of the clamped grid point
-    int xs[4], ys[4], zs[4];
-    float wx[4], wy[4], wz[4];
     float fx, fy, fz;
+    float wx[4], wy[4], wz[4];
+    unsigned xoff;      // byte offset of column x-1 (cubic) or x (trilinear)
+    unsigned yoff[4];   // byte offsets of rows y-1, y, y+1, min(y+2, ny-1)
+    unsigned zoff[4];   // byte offsets of planes z-1 .. min(z+2, nz-1)
+    bool xedge;         // x+2 == nx: the fourth column is clamped to nx-1
 };
 
-// world_to_cell_grid + clamp to [1, dim-2] (warping.h:31-36, 54-63).
 __device__ __forceinline__ void sample_point(const VolGeom& g, float px, float py, float pz, bool cubic, SamplePt& s) {
-    const float dims[3] = {(float)g.nx, (float)g.ny, (float)g.nz};
+    const float d[3] = {(float)g.nx, (float)g.ny, (float)g.nz};
     const float p[3] = {px, py, pz};
     float gc[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float v = (dims[c] * p[c]) / dims[c] - 0.5f;
-        v = fmaxf(1.0f, fminf(dims[c] - 2.0f, v));
-        gc[c] = v;
-    }
-    s.ix = (int)gc[0]; s.iy = (int)gc[1]; s.iz = (int)gc[2];
-    s.fx = gc[0] - (float)s.ix; s.fy = gc[1] - (float)s.iy; s.fz = gc[2] - (float)s.iz;
+    for (int c = 0; c < 3; ++c) gc[c] = __builtin_amdgcn_fmed3f((d[c] * p[c]) / d[c] - 0.5f, 1.0f, d[c] - 2.0f);
+    const int ix = (int)gc[0], iy = (int)gc[1], iz = (int)gc[2];
+    s.fx = gc[0] - (float)ix;
+    s.fy = gc[1] - (float)iy;
+    s.fz = gc[2] - (float)iz;
+    const unsigned ny4 = 4u * (unsigned)g.nx, nz4 = 4u * (unsigned)(g.nx * g.ny);
     if (cubic) {
-        const int n[3] = {g.nx, g.ny, g.nz};
-        const int base[3] = {s.ix, s.iy, s.iz};
-        const float f[3] = {s.fx, s.fy, s.fz};
-        int* idx[3] = {s.xs, s.ys, s.zs};
-        float* w[3] = {s.wx, s.wy, s.wz};
+        cr_weights(s.fx, s.wx);
+        cr_weights(s.fy, s.wy);
+        cr_weights(s.fz, s.wz);
+        s.xoff = 4u * (unsigned)(ix - 1);
+        s.xedge = ix + 2 >= g.nx;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int b = base[c];
-            idx[c][0] = b - 1 < 0 ? 0 : b - 1;
-            idx[c][1] = b;
-            idx[c][2] = f[c] > 0.0f ? b + 1 : b;
-            idx[c][3] = b + 2 >= n[c] ? n[c] - 1 : b + 2;
-            cr_weights(f[c], w[c]);
+        for (int k = 0; k < 4; ++k) {
+            s.yoff[k] = __umul24((unsigned)min(iy - 1 + k, g.ny - 1), ny4);
+            s.zoff[k] = __umul24((unsigned)min(iz - 1 + k, g.nz - 1), nz4);
+        }
+    } else {
+        s.xoff = 4u * (unsigned)ix;
+        s.xedge = false;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            s.yoff[k] = __umul24((unsigned)(iy + k), ny4);
+            s.zoff[k] = __umul24((unsigned)(iz + k), nz4);
         }
     }
 }
 
-// sample_image (warping.h:49-88): clamped Catmull-Rom, bounded by the min/max
-// of the 8 corners of the cell; or trilinear at the same point.
-__device__ __forceinline__ float sample_img(const gfloat* __restrict__ img, const VolGeom& g, const SamplePt& s,
+// sample_image (warping.h:49-88) on channel plane `plane_off` (bytes) of an
+// image buffer: clamped Catmull-Rom bounded by the min/max of the 8 corners
+// of the cell (order x, y, z as CImg), or trilinear at the same point.
+__device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned plane_off, const SamplePt& s,
                                             bool cubic) {
-    const int sy = g.nx, sz = g.nx * g.ny;
-    const gfloat* c0 = img + (s.ix + sy * s.iy + sz * s.iz);
-    const float a000 = c0[0], a100 = c0[1], a010 = c0[sy], a110 = c0[sy + 1];
-    const float a001 = c0[sz], a101 = c0[sz + 1], a011 = c0[sz + sy], a111 = c0[sz + sy + 1];
     if (!cubic) {
+        float2 r[2][2];
+#pragma unroll
+        for (int kz = 0; kz < 2; ++kz)
+#pragma unroll
+            for (int ky = 0; ky < 2; ++ky)
+                r[kz][ky] = __builtin_bit_cast(
+                    float2, __builtin_amdgcn_raw_buffer_load_b64(rs, s.xoff + s.yoff[ky] + s.zoff[kz], plane_off, 0));
         const float gx = 1.0f - s.fx, gy = 1.0f - s.fy;
-        const float c00 = a000 * gx + a100 * s.fx, c10 = a010 * gx + a110 * s.fx;
-        const float c01 = a001 * gx + a101 * s.fx, c11 = a011 * gx + a111 * s.fx;
+        const float c00 = r[0][0].x * gx + r[0][0].y * s.fx, c10 = r[0][1].x * gx + r[0][1].y * s.fx;
+        const float c01 = r[1][0].x * gx + r[1][0].y * s.fx, c11 = r[1][1].x * gx + r[1][1].y * s.fx;
         const float e0 = c00 * gy + c10 * s.fy, e1 = c01 * gy + c11 * s.fy;
         return e0 * (1.0f - s.fz) + e1 * s.fz;
     }
-    const float mn = fminf(fminf(fminf(a000, a100), fminf(a010, a110)), fminf(fminf(a001, a101), fminf(a011, a111)));
-    const float mx = fmaxf(fmaxf(fmaxf(a000, a100), fmaxf(a010, a110)), fmaxf(fmaxf(a001, a101), fmaxf(a011, a111)));
+    float mn = INFINITY, mx = -INFINITY;
     float acc_z = 0.0f;
 #pragma unroll
     for (int kz = 0; kz < 4; ++kz) {
         float acc_y = 0.0f;
 #pragma unroll
         for (int ky = 0; ky < 4; ++ky) {
-            const gfloat* row = img + (size_t)sy * s.ys[ky] + (size_t)sz * s.zs[kz];
-            const float r = s.wx[0] * row[s.xs[0]] + s.wx[1] * row[s.xs[1]] + s.wx[2] * row[s.xs[2]] +
-                            s.wx[3] * row[s.xs[3]];
-            acc_y = fmaf(s.wy[ky], r, acc_y);
+            float4 row = __builtin_bit_cast(
+                float4, __builtin_amdgcn_raw_buffer_load_b128(rs, s.xoff + s.yoff[ky] + s.zoff[kz], plane_off, 0));
+            if (s.xedge) row.w = row.z;
+            if ((ky == 1 || ky == 2) && (kz == 1 || kz == 2)) {
+                mn = fminf(mn, fminf(row.y, row.z));
+                mx = fmaxf(mx, fmaxf(row.y, row.z));
+            }
+            const float rv = s.wx[0] * row.x + s.wx[1] * row.y + s.wx[2] * row.z + s.wx[3] * row.w;
+            acc_y = fmaf(s.wy[ky], rv, acc_y);
         }
         acc_z = fmaf(s.wz[kz], acc_y, acc_z);
     }
@@ -467,14 +508,18 @@ __device__ __forceinline__ float sample_img(const gfloat* __restrict__ img, cons
 // trace (warping.h:100-114) from the cell centre of (i,j,k) with trace step
 // tdt (= -dt of warp), then the image sample.  The first velocity sample sits
 // exactly on grid node clamp(i,1,n-2), where the clamped cubic returns the node
-// value, so it is read directly.
-__device__ __forceinline__ float warp_voxel(const gfloat* __restrict__ img, const gfloat* __restrict__ uvw,
-                                            const VolGeom& g, int i, int j, int k, float tdt, bool cubic) {
-    const size_t V = (size_t)g.nx * g.ny * g.nz;
+// value, so it is read directly.  The three velocity planes share one buffer.
+__device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, const VolGeom& g, int i, int j, int k,
+                                            float tdt, bool cubic) {
+    const unsigned V = (unsigned)(g.nx * g.ny * g.nz);
+    const __amdgpu_buffer_rsrc_t rs_img = __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, (int)(4u * V), kBufWord3);
+    const __amdgpu_buffer_rsrc_t rs_vel =
+        __builtin_amdgcn_make_buffer_rsrc((void*)uvw, (short)0, (int)(12u * V), kBufWord3);
     const int ci = min(max(i, 1), g.nx - 2), cj = min(max(j, 1), g.ny - 2), ck = min(max(k, 1), g.nz - 2);
     const size_t nq = (size_t)ci + (size_t)g.nx * ((size_t)cj + (size_t)g.ny * ck);
+    const gfloat* u = gptr(uvw);
     const float p[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
-    float v1[3] = {uvw[nq], uvw[V + nq], uvw[2 * V + nq]};
+    const float v1[3] = {u[nq], u[V + nq], u[2 * (size_t)V + nq]};
     float p2[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p2[c] = p[c] + tdt * v1[c];
@@ -482,7 +527,7 @@ __device__ __forceinline__ float warp_voxel(const gfloat* __restrict__ img, cons
     sample_point(g, p2[0], p2[1], p2[2], cubic, s);
     float v2[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v2[c] = sample_img(uvw + c * V, g, s, cubic);
+    for (int c = 0; c < 3; ++c) v2[c] = sample_img(rs_vel, 4u * V * (unsigned)c, s, cubic);
     float q[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -490,7 +535,7 @@ __device__ __forceinline__ float warp_voxel(const gfloat* __restrict__ img, cons
         q[c] = p2[c] + 0.5f * (p[c] - p0);
     }
     sample_point(g, q[0], q[1], q[2], cubic, s);
-    return sample_img(img, g, s, cubic);
+    return sample_img(rs_img, 0u, s, cubic);
 }
 
 __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, const float* __restrict__ src,
@@ -500,7 +545,7 @@ __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, con
     const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    out[q] = warp_voxel(gptr(src), gptr(uvw), g, x, y, z, tdt, cubic != 0);
+    out[q] = warp_voxel(src, uvw, g, x, y, z, tdt, cubic != 0);
 }
 
 // Dual step of frame f (ST_Tomography.cpp:502-534); grid.z = z + nz*(f - f0).
@@ -521,7 +566,7 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
     gfloat* Ytp = gptr(B.ytp);
     gfloat* Ybc = gptr(B.ybc);
     // m_backward_warp = warp(u_f, X_bar[f+1], dt = -1): trace step +1 (:503)
-    const float bw = warp_voxel(xb1, gptr(B.flow), g, x, y, z, 1.0f, pc.interp == 0);
+    const float bw = warp_voxel(B1.xbar, B.flow, g, x, y, z, 1.0f, pc.interp == 0);
     const float xq = xb0[q];
     float ytp = Ytp[q] + pc.sigma * (xb1[q] - xq);  // (:504)
     float ybc = Ybc[q] + pc.sigma * (bw - xq);      // (:505)
@@ -570,7 +615,7 @@ __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__
         xv = xv - pc.tau * (-Ybc[q]);
     } else {                // (:546-555): forward warp of Y_bc[f-1] with u_{f-1}, trace step -1
         const FrameBufs& Bm = fb[f - 1];
-        const float fw = warp_voxel(gptr(Bm.ybc), gptr(Bm.flow), g, x, y, z, -1.0f, pc.interp == 0);
+        const float fw = warp_voxel(Bm.ybc, Bm.flow, g, x, y, z, -1.0f, pc.interp == 0);
         xv = xv - pc.tau * (gptr(Bm.ytp)[q] - Ytp[q]);
         xv = xv - pc.tau * (fw - Ybc[q]);
     }

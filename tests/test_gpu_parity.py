@@ -103,21 +103,31 @@ def test_bp_ray_matches_oracle():
 
 
 def test_fp_vjp_dot_product():
-    """<A x, y> == <x, A^T y> for the matched ray-driven transpose."""
+    """<A x, y> == <x, A^T y> for the matched ray-driven transpose: relative
+    to ||A x|| ||y|| for signed inputs (the dot product itself cancels), and
+    relative to the products themselves for non-negative inputs."""
     print("[fp-vjp] 24^3 det 40x40", flush=True)
     rng = np.random.default_rng(9)
     vol, det = (24, 24, 24), (40, 40)
     g, ctx = _ctx(vol, det, P=(2,))
     ctx.set_views(0, [10.0, 250.0])
-    x = rng.standard_normal(int(np.prod(vol))).astype(np.float32)
-    y = rng.standard_normal(det[0] * det[1]).astype(np.float32)
-    ctx.set_volume(0, x)
-    for v in range(2):
-        ax, _, _ = ctx.forward_project(0, v)
-        aty = ctx.fp_vjp(0, v, y)
-        lhs = float(np.dot(ax.astype(np.float64), y.astype(np.float64)))
-        rhs = float(np.dot(x.astype(np.float64), aty.astype(np.float64)))
-        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs)) + 1e-6, (lhs, rhs)
+    for signed in (True, False):
+        if signed:
+            x = rng.standard_normal(int(np.prod(vol))).astype(np.float32)
+            y = rng.standard_normal(det[0] * det[1]).astype(np.float32)
+        else:
+            x = rng.random(int(np.prod(vol)), dtype=np.float32)
+            y = rng.random(det[0] * det[1], dtype=np.float32)
+        ctx.set_volume(0, x)
+        for v in range(2):
+            ax, _, _ = ctx.forward_project(0, v)
+            aty = ctx.fp_vjp(0, v, y)
+            lhs = float(np.dot(ax.astype(np.float64), y.astype(np.float64)))
+            rhs = float(np.dot(x.astype(np.float64), aty.astype(np.float64)))
+            scale = (np.linalg.norm(ax.astype(np.float64)) * np.linalg.norm(y.astype(np.float64))
+                     if signed else max(abs(lhs), abs(rhs)))
+            print(f"  signed={signed} view={v} <Ax,y>={lhs:.9g} <x,A^Ty>={rhs:.9g} rel={abs(lhs - rhs) / scale:.2e}")
+            assert abs(lhs - rhs) <= 1e-6 * scale, (lhs, rhs, scale)
     ctx.close()
 
 
