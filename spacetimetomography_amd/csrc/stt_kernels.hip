@@ -69,6 +69,8 @@ __device__ __forceinline__ void produce_ray(const ViewRec& v, const VolGeom& g, 
 // Global-address-space views of pointers read from work tables, so loads and
 // atomics compile to global_* instead of flat_* instructions.
 typedef __attribute__((address_space(1))) float gfloat;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) f32x4 gf32x4;
 __device__ __forceinline__ gfloat* gptr(float* p) { return (gfloat*)p; }
 __device__ __forceinline__ const gfloat* gptr(const float* p) { return (const gfloat*)p; }
 
@@ -98,25 +100,43 @@ __device__ __forceinline__ VolView vol_view(const float* vol, const VolGeom& g) 
 // T_VolumeAccess::GetScalarValue trilinear (SingleScalarVolume.core.h:87-146):
 // clamp to [0, dims-1], taps at floor and floor+1.  The volume allocation
 // carries a zero tail of nx*ny + nx + 1 floats, so the +1 taps need no clamp:
-// where the reference clamps a tap, its weight is exactly 0.
-__device__ __forceinline__ float sample_tri(const VolView& v, float px, float py, float pz) {
+// where the reference clamps a tap, its weight is exactly 0.  Split into the
+// tap fetch and the interpolation so the caster can keep the next sample's
+// loads in flight while it interpolates the current one.
+struct Taps {
+    float2 r00, r10, r01, r11;  // (x, x+1) pairs of rows (y, z), (y+1, z), (y, z+1), (y+1, z+1)
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ void fetch_taps(const VolView& v, float px, float py, float pz, Taps& T) {
     px = __builtin_amdgcn_fmed3f(px, 0.0f, v.bx);
     py = __builtin_amdgcn_fmed3f(py, 0.0f, v.by);
     pz = __builtin_amdgcn_fmed3f(pz, 0.0f, v.bz);
     const unsigned ix = (unsigned)px, iy = (unsigned)py, iz = (unsigned)pz;
-    const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py), fz = __builtin_amdgcn_fractf(pz);
+    T.fx = __builtin_amdgcn_fractf(px);
+    T.fy = __builtin_amdgcn_fractf(py);
+    T.fz = __builtin_amdgcn_fractf(pz);
     const unsigned off = (ix << 2) + __umul24(iy, v.nx4) + __umul24(iz, v.sxy4);
-    const float2 r00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, 0, 0));
-    const float2 r10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.nx4, 0));
-    const float2 r01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxy4, 0));
-    const float2 r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxynx4, 0));
-    const float a = fmaf(fx, r00.y - r00.x, r00.x);
-    const float b = fmaf(fx, r10.y - r10.x, r10.x);
-    const float c = fmaf(fx, r01.y - r01.x, r01.x);
-    const float d = fmaf(fx, r11.y - r11.x, r11.x);
-    const float ab = fmaf(fy, b - a, a);
-    const float cd = fmaf(fy, d - c, c);
-    return fmaf(fz, cd - ab, ab);
+    T.r00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, 0, 0));
+    T.r10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.nx4, 0));
+    T.r01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxy4, 0));
+    T.r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxynx4, 0));
+}
+
+__device__ __forceinline__ float lerp_taps(const Taps& T) {
+    const float a = fmaf(T.fx, T.r00.y - T.r00.x, T.r00.x);
+    const float b = fmaf(T.fx, T.r10.y - T.r10.x, T.r10.x);
+    const float c = fmaf(T.fx, T.r01.y - T.r01.x, T.r01.x);
+    const float d = fmaf(T.fx, T.r11.y - T.r11.x, T.r11.x);
+    const float ab = fmaf(T.fy, b - a, a);
+    const float cd = fmaf(T.fy, d - c, c);
+    return fmaf(T.fz, cd - ab, ab);
+}
+
+__device__ __forceinline__ float sample_tri(const VolView& v, float px, float py, float pz) {
+    Taps T;
+    fetch_taps(v, px, py, pz, T);
+    return lerp_taps(T);
 }
 
 // Scatter of the 8 trilinear weights (SingleScalarVolume.core.h:99-105, 128-143)
@@ -199,14 +219,33 @@ __device__ __forceinline__ float cast_ray(const VolView& vol, float* contrib, fl
             py = r.o[1] + r.d[1] * t;
             pz = r.o[2] + r.d[2] * t;
         }
-        while (t < tb && n < g.max_samples) {
-            if (SCATTER) scatter_tri(contrib, wsum, g, px, py, pz, sval);
-            else density = fmaf(sample_tri(vol, px, py, pz), sd, density);
-            ++n;
-            t += sd;
-            px += step[0];
-            py += step[1];
-            pz += step[2];
+        if (SCATTER) {
+            while (t < tb && n < g.max_samples) {
+                scatter_tri(contrib, wsum, g, px, py, pz, sval);
+                ++n;
+                t += sd;
+                px += step[0];
+                py += step[1];
+                pz += step[2];
+            }
+        } else if (t < tb) {
+            // one sample ahead: the taps of sample k+1 are in flight while
+            // sample k is interpolated
+            Taps cur;
+            fetch_taps(vol, px, py, pz, cur);
+            while (n < g.max_samples) {
+                ++n;
+                t += sd;
+                px += step[0];
+                py += step[1];
+                pz += step[2];
+                const bool more = t < tb;
+                Taps nxt;
+                if (more) fetch_taps(vol, px, py, pz, nxt);
+                density = fmaf(lerp_taps(cur), sd, density);
+                if (!more) break;
+                cur = nxt;
+            }
         }
         if (tMax[0] < tMax[1]) {
             if (tMax[0] < tMax[2]) tMax[0] += tDelta[0];
@@ -320,8 +359,7 @@ __device__ __forceinline__ void cr_weights(float t, float w[4]) {
 // Detector position of a voxel centre with the reference's arithmetic
 // (ST_Tomography.cpp:698-703): float matrix-vector product without
 // contraction, IEEE division, m_halfFrustumWidth/m_ds in double.
-__device__ __noinline__ void exact_detector_position(const ViewRec& v, float fx, float fy, float fz, float& pxf,
-                                                     float& pyf) {
+__device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float fx, float fy, float fz) {
 #pragma clang fp contract(off)
     const float* M = v.mvv2m;
     const float e0 = ((M[0] * fx + M[1] * fy) + M[2] * fz) + M[3];
@@ -331,65 +369,103 @@ __device__ __noinline__ void exact_detector_position(const ViewRec& v, float fx,
     const float ty = (e1 / e2) * v.sdd;
     const float ux = (float)(v.halfW - (double)tx);
     const float uy = (float)(v.halfH - (double)ty);
-    pxf = (float)((double)ux / v.ds);
-    pyf = (float)((double)uy / v.ds);
+    return make_float2((float)((double)ux / v.ds), (float)((double)uy / v.ds));
 }
 
 // Voxel-driven backprojection (ST_Tomography.cpp:687-731): project the voxel
 // centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
-// clamp [0, G_Max].  Atomic-free: one thread per voxel, 64x4 voxels of one
-// z-slice per block (x fastest, coalesced RMW of the volume); grid.z = z + nz*item.
-// The 4x4 detector window is read as four 16-byte rows through a raw buffer.
+// clamp [0, G_Max].  Atomic-free: each thread owns 4 consecutive x-voxels
+// (16-byte volume load/store, so enough bytes are in flight to stream HBM);
+// a 256-thread block covers 1024 voxels of one (y, z) slab row set, with
+// grid.z = z + nz*item.  The 4x4 detector window is read as four 16-byte rows
+// through a raw buffer.
+constexpr int kBpVox = 4;
+
+__device__ __forceinline__ float bp_update(const ViewRec& v, const VolGeom& g, const PdConst& pc,
+                                           __amdgpu_buffer_rsrc_t rs, float fx, float fy, float fz, float ex,
+                                           float ey, float ez, float xv, bool& touched) {
+    const float r = __builtin_amdgcn_rcpf(ez);
+    float pxf = fmaf(-ex, r, v.bp[12]);
+    float pyf = fmaf(-ey, r, v.bp[13]);
+    // Within 0.01 px of the detector window's edges the update/skip decision is
+    // taken with the reference's own arithmetic (rare, divergent).
+    const float hx = (float)(g.W - 2), hy = (float)(g.H - 2);
+    if (fabsf(pxf - 2.0f) < 0.01f || fabsf(pxf - hx) < 0.01f || fabsf(pyf - 2.0f) < 0.01f || fabsf(pyf - hy) < 0.01f) {
+        const float2 e = exact_detector_position(v, fx, fy, fz);
+        pxf = e.x;
+        pyf = e.y;
+    }
+    const int px = (int)pxf, py = (int)pyf;
+    touched = !(px < 2 || px >= g.W - 2 || py < 2 || py >= g.H - 2);
+    if (!touched) return xv;
+    const float dx = pxf - (float)px, dy = pyf - (float)py;
+    const unsigned off = 4u * (unsigned)((py - 1) * g.W + (px - 1));
+    const unsigned W4 = 4u * (unsigned)g.W;
+    float wy[4], wx[4];
+    cr_weights(dy, wy);
+    cr_weights(dx, wx);
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const float4 row = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, b * W4, 0));
+        acc.x = fmaf(wy[b], row.x, acc.x);
+        acc.y = fmaf(wy[b], row.y, acc.y);
+        acc.z = fmaf(wy[b], row.z, acc.z);
+        acc.w = fmaf(wy[b], row.w, acc.w);
+    }
+    const float gv = wx[0] * acc.x + wx[1] * acc.y + wx[2] * acc.z + wx[3] * acc.w;
+    return __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xv), 0.0f, pc.gmax);
+}
+
 __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ items, VolGeom g, PdConst pc) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * kBpVox;
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z % g.nz;
     const WorkItem& it = items[blockIdx.z / g.nz];
-    if (x >= g.nx || y >= g.ny) return;
+    if (x0 >= g.nx || y >= g.ny) return;
     const ViewRec& v = *it.view;
-    const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    const size_t q0 = (size_t)x0 + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)it.corr, (short)0, (int)g.det_bytes, kBufWord3);
     const float* A = v.bp;
-    const float ex = fmaf(A[0], fx, fmaf(A[1], fy, fmaf(A[2], fz, A[3])));
-    const float ey = fmaf(A[4], fx, fmaf(A[5], fy, fmaf(A[6], fz, A[7])));
-    const float ez = fmaf(A[8], fx, fmaf(A[9], fy, fmaf(A[10], fz, A[11])));
-    const float r = __builtin_amdgcn_rcpf(ez);
-    float pxf = fmaf(-ex, r, A[12]);
-    float pyf = fmaf(-ey, r, A[13]);
-    // Within 0.01 px of the detector window's edges the update/skip decision is
-    // taken with the reference's own arithmetic (rare, divergent).
-    const float lo = 2.0f, hx = (float)(g.W - 2), hy = (float)(g.H - 2);
-    if (fabsf(pxf - lo) < 0.01f || fabsf(pxf - hx) < 0.01f || fabsf(pyf - lo) < 0.01f || fabsf(pyf - hy) < 0.01f)
-        exact_detector_position(v, fx, fy, fz, pxf, pyf);
-    const int px = (int)pxf, py = (int)pyf;
-    const bool inside = !(px < 2 || px >= g.W - 2 || py < 2 || py >= g.H - 2);
-    if (!inside && !it.last) return;
+    const float fy = (float)y, fz = (float)z;
+    // detector-space row of x = x0, then one column of MV*V2M per voxel step
+    float ex = fmaf(A[0], (float)x0, fmaf(A[1], fy, fmaf(A[2], fz, A[3])));
+    float ey = fmaf(A[4], (float)x0, fmaf(A[5], fy, fmaf(A[6], fz, A[7])));
+    float ez = fmaf(A[8], (float)x0, fmaf(A[9], fy, fmaf(A[10], fz, A[11])));
     gfloat* vol = gptr(it.vol);
-    float xv = vol[q];
-    if (inside) {
-        const float dx = pxf - (float)px, dy = pyf - (float)py;
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void*)it.corr, (short)0, (int)g.det_bytes, kBufWord3);
-        const unsigned off = 4u * (unsigned)((py - 1) * g.W + (px - 1));
-        const unsigned W4 = 4u * (unsigned)g.W;
-        float wy[4], wx[4];
-        cr_weights(dy, wy);
-        cr_weights(dx, wx);
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool full = x0 + kBpVox <= g.nx && (g.nx & 3) == 0;
+    float xs[kBpVox];
+    if (full) {
+        const f32x4 q = *(const gf32x4*)(vol + q0);
+        xs[0] = q.x; xs[1] = q.y; xs[2] = q.z; xs[3] = q.w;
+    } else {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const float4 row = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, b * W4, 0));
-            acc.x = fmaf(wy[b], row.x, acc.x);
-            acc.y = fmaf(wy[b], row.y, acc.y);
-            acc.z = fmaf(wy[b], row.z, acc.z);
-            acc.w = fmaf(wy[b], row.w, acc.w);
-        }
-        const float gv = wx[0] * acc.x + wx[1] * acc.y + wx[2] * acc.z + wx[3] * acc.w;
-        xv = fmaf(pc.alpha, gv, xv);
-        xv = __builtin_amdgcn_fmed3f(xv, 0.0f, pc.gmax);
+        for (int i = 0; i < kBpVox; ++i) xs[i] = (x0 + i < g.nx) ? vol[q0 + i] : 0.0f;
     }
-    if (it.last) xv = finalize_voxel(it, g, pc, x, y, z, q, xv);
-    vol[q] = xv;
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < kBpVox; ++i) {
+        bool touched = false;
+        if (x0 + i < g.nx) xs[i] = bp_update(v, g, pc, rs, (float)(x0 + i), fy, fz, ex, ey, ez, xs[i], touched);
+        any |= touched;
+        ex += A[0];
+        ey += A[4];
+        ez += A[8];
+    }
+    if (it.last) {
+#pragma unroll
+        for (int i = 0; i < kBpVox; ++i)
+            if (x0 + i < g.nx) xs[i] = finalize_voxel(it, g, pc, x0 + i, y, z, q0 + i, xs[i]);
+    } else if (!any) {
+        return;
+    }
+    if (full) {
+        *(gf32x4*)(vol + q0) = f32x4{xs[0], xs[1], xs[2], xs[3]};
+    } else {
+#pragma unroll
+        for (int i = 0; i < kBpVox; ++i)
+            if (x0 + i < g.nx) vol[q0 + i] = xs[i];
+    }
 }
 
 // Ray-BP normalisation (ST_Tomography.cpp:746-762): x += contrib/wsum with the
@@ -662,7 +738,8 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s) {
-    hipLaunchKernelGGL(k_bp_voxel, vol_grid(g, nitems), dim3(256), 0, s, items, g, pc);
+    const dim3 grid((g.nx + 64 * kBpVox - 1) / (64 * kBpVox), (g.ny + 3) / 4, g.nz * nitems);
+    hipLaunchKernelGGL(k_bp_voxel, grid, dim3(256), 0, s, items, g, pc);
     return hipGetLastError();
 }
 
