@@ -64,13 +64,13 @@ struct stt_ctx {
     hipStream_t stream = nullptr;
     VolGeom vg{};
     PdConst pc{};
-    size_t V = 0, Vpad = 0, D = 0;
+    size_t V = 0, Vpad = 0, VpadT = 0, D = 0;
     float gmax = 10.0f;
     bool gmax_from_data = true;
     bool pd_ready = false;
 
     // device buffers
-    std::vector<float*> x, xbar, xprev, ytv, ytp, ybc, flow, b, raylen_f, corr_f, contrib_f, wsum_f;
+    std::vector<float*> x, xT, xbar, xprev, ytv, ytp, ybc, flow, b, raylen_f, corr_f, contrib_f, wsum_f;
     float* raylen_shared = nullptr;
     float* zero_flow = nullptr;
     ViewRec* views_dev = nullptr;
@@ -142,6 +142,7 @@ struct stt_ctx {
         WorkItem it{};
         it.view = views_dev + view_off[f] + view;
         it.vol = x[f];
+        it.volT = xT[f];
         it.b = b[f] + (size_t)view * D;
         it.raylen = shared_raylen ? raylen_shared : raylen_f[f];
         it.corr = corr_f[f];
@@ -258,6 +259,7 @@ struct stt_ctx {
             mark(4);
             for (int k = 0; k < maxP; ++k) {
                 const WorkItem* it = items_batch + batch_off[k];
+                HIPCHK(launch_transpose_xz(it, batch_cnt[k], vg, stream));
                 HIPCHK(launch_fp(FP_MODE_SART, it, batch_cnt[k], vg, pc.factor, stream));
                 mark(1);
                 int rc = launch_bp_for(it, batch_cnt[k], ray_bp);
@@ -273,6 +275,7 @@ struct stt_ctx {
                 mark(4);
                 for (int k = 0; k < P[f]; ++k) {
                     const WorkItem* it = items_plain + off + k;
+                    HIPCHK(launch_transpose_xz(it, 1, vg, stream));
                     HIPCHK(launch_fp(FP_MODE_SART, it, 1, vg, pc.factor, stream));
                     mark(1);
                     int rc = launch_bp_for(it, 1, ray_bp);
@@ -320,7 +323,7 @@ int valid_geometry(const stt_geometry* g) {
     const double v = (double)g->vol[0] * g->vol[1] * g->vol[2];
     if ((v + tail_pad(g->vol)) * 4.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "a volume must stay below 4 GiB");
     if ((double)g->det[0] * g->det[1] * 4.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "detector too large");
-    if ((double)g->vol[0] * g->vol[1] * 4.0 >= 16777216.0)
+    if ((double)g->vol[0] * g->vol[1] * 4.0 >= 16777216.0 || (double)g->vol[2] * g->vol[1] * 4.0 >= 16777216.0)
         return fail(STT_ERR_INVALID, "a volume slice must stay below 4M voxels (24-bit offsets)");
     return STT_OK;
 }
@@ -389,6 +392,8 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
         return bail(fail(STT_ERR_HIP, "hipStreamCreate failed"));
     c->V = (size_t)g->vol[0] * g->vol[1] * g->vol[2];
     c->Vpad = c->V + tail_pad(g->vol);
+    const int volT[3] = {g->vol[2], g->vol[1], g->vol[0]};
+    c->VpadT = c->V + tail_pad(volT);
     c->D = (size_t)g->det[0] * g->det[1];
     c->vg.nx = g->vol[0];
     c->vg.ny = g->vol[1];
@@ -401,10 +406,12 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     c->vg.bz = (float)(g->vol[2] - 1);
     c->vg.max_samples = 2 * (g->vol[0] + g->vol[1] + g->vol[2]) + 16;
     c->vg.vol_bytes = (unsigned)(c->Vpad * sizeof(float));
+    c->vg.volT_bytes = (unsigned)(c->VpadT * sizeof(float));
     c->vg.det_bytes = (unsigned)((size_t)g->det[0] * g->det[1] * sizeof(float));
     c->derive_consts();
     const int F = nframes;
     c->x.resize(F + 1);
+    c->xT.resize(F + 1);
     c->xbar.resize(F + 1);
     c->xprev.resize(F + 1);
     c->ytv.resize(F + 1);
@@ -416,6 +423,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     c->corr_f.resize(F);
     for (int f = 0; f <= F; ++f) {
         if ((rc = c->alloc(&c->x[f], c->Vpad))) return bail(rc);
+        if ((rc = c->alloc(&c->xT[f], c->VpadT))) return bail(rc);
         if ((rc = c->alloc(&c->xbar[f], c->V))) return bail(rc);
         if ((rc = c->alloc(&c->xprev[f], c->V))) return bail(rc);
         if ((rc = c->alloc(&c->ytv[f], 3 * c->V))) return bail(rc);
@@ -721,6 +729,7 @@ int stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n) {
     for (int i = 0; i < n; ++i) {
         int rc = ctx->upload_scratch({items[i]});
         if (rc) return rc;
+        HIPCHK(launch_transpose_xz(ctx->items_scratch, 1, ctx->vg, ctx->stream));
         HIPCHK(launch_fp(FP_MODE_SART, ctx->items_scratch, 1, ctx->vg, ctx->pc.factor, ctx->stream));
         rc = ctx->launch_bp_for(ctx->items_scratch, 1, ray_bp);
         if (rc) return rc;

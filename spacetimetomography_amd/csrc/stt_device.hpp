@@ -23,7 +23,11 @@ struct alignas(256) ViewRec {
     // reference multiplies them, float m_sdd, double m_halfFrustumWidth/Height
     // and m_ds (Scene.h:111-116).
     float mvv2m[12];
-    float sdd, pad0;
+    float sdd;
+    // Plane marching of the projector: bits 0-1 dominant axis D of the view's
+    // central ray (0 x, 1 y, 2 z), bit 2 set when that ray runs towards -D,
+    // bit 3 set when the view samples the transposed (z-fastest) volume copy.
+    int march;
     double halfW, halfH, ds;
 };
 static_assert(sizeof(ViewRec) == 256, "ViewRec must stay one 256-byte record");
@@ -36,6 +40,7 @@ struct VolGeom {
     float bx, by, bz;    // CroppingBounds upper faces dims-1 (TypicalRayProducer.core.h:241-244)
     int max_samples;     // hard cap of the sample loop (never reached by a valid ray)
     unsigned vol_bytes;  // bytes of one padded volume allocation (buffer range)
+    unsigned volT_bytes; // bytes of one padded transposed (z-fastest) volume
     unsigned det_bytes;  // bytes of one detector image
 };
 
@@ -54,6 +59,7 @@ struct PdConst {
 struct WorkItem {
     const ViewRec* view;
     float* vol;          // x[f]
+    float* volT;         // transposed copy of x[f] (z fastest) for x-dominant views
     const float* b;      // measured projection of this view (W*H)
     float* raylen;       // stale ray-length buffer (read/write)
     float* corr;         // correction image / ray-BP input (W*H)

@@ -191,6 +191,18 @@ ViewRec build_view(const stt_geometry& g, const float q[4]) {
     rec.eye[2] = p2m.e[10] / p2m.e[11];
     rec.eye[3] = 1.0f;
     apply(m2v, rec.eye, rec.o);
+    // dominant axis of the ray from the source to the volume centre (voxel units)
+    int dom = 0;
+    float dmax = -1.0f, dsgn = 1.0f;
+    for (int c = 0; c < 3; ++c) {
+        const float dc = 0.5f * (g.vol[c] - 1) - rec.o[c];
+        if (std::fabs(dc) > dmax) {
+            dmax = std::fabs(dc);
+            dom = c;
+            dsgn = dc < 0 ? -1.0f : 1.0f;
+        }
+    }
+    rec.march = dom | (dsgn < 0 ? 4 : 0) | (dom == 0 ? 8 : 0);
     for (int c = 0; c < 3; ++c) rec.m2v[c] = m2v.e[5 * c];
     rec.m2v[3] = 1.0f;
     rec.halfW = halfW;

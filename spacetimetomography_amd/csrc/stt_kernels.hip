@@ -139,6 +139,37 @@ __device__ __forceinline__ float sample_tri(const VolView& v, float px, float py
     return lerp_taps(T);
 }
 
+// Raw buffer over the transposed copy (index z + nz*(y + ny*x), zero tail of
+// nz*ny + nz + 1 floats): the contiguous tap pair runs along z.
+__device__ __forceinline__ VolView vol_view_t(const float* volT, const VolGeom& g) {
+    VolView v;
+    v.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)volT, (short)0, (int)g.volT_bytes, kBufWord3);
+    v.nx4 = 4u * (unsigned)g.nz;
+    v.sxy4 = 4u * (unsigned)(g.nz * g.ny);
+    v.sxynx4 = v.sxy4 + v.nx4;
+    v.bx = g.bx;
+    v.by = g.by;
+    v.bz = g.bz;
+    return v;
+}
+
+// Same trilinear sample from the transposed copy: interpolation runs along
+// z, then y, then x (fractions placed so lerp_taps applies unchanged).
+__device__ __forceinline__ void fetch_taps_t(const VolView& v, float px, float py, float pz, Taps& T) {
+    px = __builtin_amdgcn_fmed3f(px, 0.0f, v.bx);
+    py = __builtin_amdgcn_fmed3f(py, 0.0f, v.by);
+    pz = __builtin_amdgcn_fmed3f(pz, 0.0f, v.bz);
+    const unsigned ix = (unsigned)px, iy = (unsigned)py, iz = (unsigned)pz;
+    T.fx = __builtin_amdgcn_fractf(pz);
+    T.fy = __builtin_amdgcn_fractf(py);
+    T.fz = __builtin_amdgcn_fractf(px);
+    const unsigned off = (iz << 2) + __umul24(iy, v.nx4) + __umul24(ix, v.sxy4);
+    T.r00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, 0, 0));
+    T.r10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.nx4, 0));
+    T.r01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxy4, 0));
+    T.r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxynx4, 0));
+}
+
 // Scatter of the 8 trilinear weights (SingleScalarVolume.core.h:99-105, 128-143)
 // with hardware fp32 atomics.  Accumulators carry the same zero tail.
 __device__ __forceinline__ void scatter_tri(float* contrib, float* wsum, const VolGeom& g, float px, float py,
@@ -344,6 +375,220 @@ __device__ __forceinline__ float finalize_voxel(const WorkItem& it, const VolGeo
     xv *= roi_in(g, pc, x, y, z) ? 1.0f : 0.0f;
     if (it.xbar && !cut_out(g, pc, x, y, z)) gptr(it.xbar)[q] = 2.0f * xv - gptr(it.xprev)[q];
     return xv;
+}
+
+// Sample generator of CastingRay (RayCaster.core.h:42-187): the entry sample,
+// then for each block segment the grid t = ceil((t1 - tnear)/sd)*sd + tnear,
+// t += sd while t < t_b, with the DDA advancing one tMax per segment.  Same
+// samples in the same order as cast_ray; it only lets the caller interleave
+// the rays of a wave by volume plane.
+struct RayGen {
+    float o[3], d[3], step[3];
+    float tnear, tfar;
+    float tMax[3], tDelta[3];
+    float t1, tb, t;
+    float pos[3];
+    int segs;
+    bool in_seg, entry;
+};
+
+__device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGeom& g) {
+    const int dimv[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        G.o[c] = r.o[c];
+        G.d[c] = r.d[c];
+        G.step[c] = r.d[c] * g.sd;
+    }
+    G.tnear = r.tnear;
+    G.tfar = r.tfar;
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) G.pos[c] = r.o[c] + r.d[c] * r.tnear;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float dc = r.d[c];
+            if (dc == 0.0f) {
+                G.tMax[c] = r.tfar;
+                G.tDelta[c] = 0.0f;
+            } else if (dc < 0.0f) {
+                const int blk = ((int)ceilf(G.pos[c] - 0.5f) - 1) / dimv[c];
+                G.tMax[c] = ((float)(blk * dimv[c]) + 0.5f - r.o[c]) / dc;
+                G.tDelta[c] = (float)(-dimv[c]) / dc;
+            } else {
+                const int blk = ((int)floorf(G.pos[c] - 0.5f)) / dimv[c];
+                G.tMax[c] = ((float)((blk + 1) * dimv[c]) + 0.5f - r.o[c]) / dc;
+                G.tDelta[c] = (float)dimv[c] / dc;
+            }
+        }
+    }
+    G.t1 = r.tnear;
+    G.tb = G.t = 0.0f;
+    G.segs = 0;
+    G.in_seg = false;
+    G.entry = true;
+}
+
+// Next sample position into p; false when the ray has no more samples.
+__device__ __forceinline__ bool raygen_next(RayGen& G, float sd, float p[3]) {
+    if (G.entry) {
+        G.entry = false;
+        p[0] = G.pos[0]; p[1] = G.pos[1]; p[2] = G.pos[2];
+        return true;
+    }
+    for (int guard = 0; guard < 10; ++guard) {
+        if (G.in_seg) {
+            if (G.t < G.tb) {
+                p[0] = G.pos[0]; p[1] = G.pos[1]; p[2] = G.pos[2];
+                G.t += sd;
+                G.pos[0] += G.step[0];
+                G.pos[1] += G.step[1];
+                G.pos[2] += G.step[2];
+                return true;
+            }
+            if (G.tMax[0] < G.tMax[1]) {
+                if (G.tMax[0] < G.tMax[2]) G.tMax[0] += G.tDelta[0];
+                else G.tMax[2] += G.tDelta[2];
+            } else {
+                if (G.tMax[1] < G.tMax[2]) G.tMax[1] += G.tDelta[1];
+                else G.tMax[2] += G.tDelta[2];
+            }
+            G.t1 = G.tb;
+            G.in_seg = false;
+        }
+        if (!(G.t1 < G.tfar) || G.segs >= 8) return false;
+        float tb = tmin(tmin(G.tMax[0], G.tMax[1]), G.tMax[2]);
+        if (tb > G.tfar) tb = G.tfar;
+        G.tb = tb;
+        {
+#pragma clang fp contract(off)
+            G.t = ceilf((G.t1 - G.tnear) / sd) * sd + G.tnear;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) G.pos[c] = G.o[c] + G.d[c] * G.t;
+        }
+        G.in_seg = true;
+        ++G.segs;
+    }
+    return false;
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Plane-marching projector.  A wave holds 64 rays of one detector row; the
+// wave walks the volume plane by plane along the view's dominant axis D and a
+// lane takes its next sample only once the wave has reached that sample's
+// plane.  At any step the wave's taps then sit in one or two planes, close
+// together across the transverse axis, which the memory layout makes
+// contiguous: x-fastest for z-dominant views, the transposed z-fastest copy
+// for x-dominant views.  Each ray's samples, their order and its sums are
+// exactly those of CastingRay.
+template <int MODE, bool TRANSPOSED>
+__device__ __forceinline__ float march_ray(const VolView& vv, const VolGeom& g, const Ray& r, bool hit, int march,
+                                           int& nsamp) {
+    RayGen G;
+    raygen_init(G, r, g);
+    const int D = march & 3;
+    const int dir = (march & 4) ? -1 : 1;
+    const float bD = D == 0 ? g.bx : (D == 1 ? g.by : g.bz);
+    float p[3];
+    bool have = hit && raygen_next(G, g.sd, p);
+    auto plane_of = [&](const float q[3]) -> int {
+        const float c = D == 0 ? q[0] : (D == 1 ? q[1] : q[2]);
+        return (int)__builtin_amdgcn_fmed3f(c, 0.0f, bD);
+    };
+    int pl = have ? plane_of(p) : (dir > 0 ? 0x7fffffff : -0x7fffffff);
+    int cur = dir > 0 ? wave_min_i32(pl) : wave_max_i32(pl);
+    float density = 0.0f;
+    int n = 0;
+    const int limit = 2 * (g.nx + g.ny + g.nz) + g.max_samples + 16;
+    for (int it = 0; it < limit && __any(have); ++it) {
+        if (have && (dir > 0 ? pl <= cur : pl >= cur)) {
+            Taps T;
+            if (TRANSPOSED) fetch_taps_t(vv, p[0], p[1], p[2], T);
+            else fetch_taps(vv, p[0], p[1], p[2], T);
+            density = fmaf(lerp_taps(T), g.sd, density);
+            ++n;
+            have = n < g.max_samples && raygen_next(G, g.sd, p);
+            if (have) pl = plane_of(p);
+        }
+        cur += dir;
+    }
+    nsamp = n;
+    // the entry sample is weighted like every other (RayCaster.core.h:60-61)
+    return density;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fp_plane(const WorkItem* __restrict__ items, VolGeom g, float factor) {
+    int i, j;
+    const bool valid = tile_pixel<64, 4>(g, i, j);
+    const WorkItem& it = items[blockIdx.y];
+    const ViewRec& v = *it.view;
+    const int march = v.march;
+    Ray r;
+    produce_ray(v, g, (float)(valid ? i : 0), (float)(valid ? j : 0), r);
+    const bool hit = valid && r.tnear < r.tfar;
+    int ns = 0;
+    float value;
+    if (MODE == FP_MODE_SART && (march & 8)) value = march_ray<MODE, true>(vol_view_t(it.volT, g), g, r, hit, march, ns);
+    else value = march_ray<MODE, false>(vol_view(it.vol, g), g, r, hit, march, ns);
+    if (!valid) return;
+    const int k = i + j * g.W;
+    if (MODE == FP_MODE_FP) {
+        if (hit && it.raylen) gptr(it.raylen)[k] = r.tfar - r.tnear;
+        if (it.ax) gptr(it.ax)[k] = hit ? value : 0.0f;
+        if (it.weight) gptr(it.weight)[k] = (float)ns;
+    } else {
+        // SART correction (ST_Tomography.cpp:667-676); pY is zero when read.
+#pragma clang fp contract(off)
+        float len;
+        gfloat* rl = gptr(it.raylen);
+        if (hit) {
+            len = r.tfar - r.tnear;
+            rl[k] = len;
+        } else {
+            len = rl[k];  // stale value of the last view whose ray k hit
+        }
+        const float num = factor * gptr(it.b)[k] - factor * (hit ? value : 0.0f);
+        const float den = factor * len + 1.0f;
+        gptr(it.corr)[k] = num / den;
+    }
+}
+
+// x <-> z transpose of the volumes of the x-dominant views in a work table
+// (index x + nx*(y + ny*z) -> z + nz*(y + ny*x)) through a 32x33 LDS tile.
+__global__ __launch_bounds__(256) void k_transpose_xz(const WorkItem* __restrict__ items, VolGeom g) {
+    const WorkItem& it = items[blockIdx.z / g.ny];
+    if (!(it.view->march & 8)) return;  // uniform per block
+    const int y = blockIdx.z % g.ny;
+    __shared__ float tile[32][33];
+    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const gfloat* in = gptr((const float*)it.vol);
+    gfloat* out = gptr(it.volT);
+    const size_t sxy = (size_t)g.nx * g.ny;
+#pragma unroll
+    for (int jj = ty; jj < 32; jj += 8) {
+        const int x = x0 + tx, z = z0 + jj;
+        if (x < g.nx && z < g.nz) tile[jj][tx] = in[(size_t)x + (size_t)g.nx * y + sxy * z];
+    }
+    __syncthreads();
+    const size_t szy = (size_t)g.nz * g.ny;
+#pragma unroll
+    for (int jj = ty; jj < 32; jj += 8) {
+        const int x = x0 + jj, z = z0 + tx;
+        if (x < g.nx && z < g.nz) out[(size_t)z + (size_t)g.nz * y + szy * x] = tile[tx][jj];
+    }
 }
 
 // Catmull-Rom weights of cubicInterpolate (ST_Tomography.cpp:183-186),
@@ -728,12 +973,20 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
     constexpr int TX = 16, TY = 16;
     const int nb = ((g.W + TX - 1) / TX) * ((g.H + TY - 1) / TY);
     dim3 grid(nb, nitems);
+    const int nbp = ((g.W + 63) / 64) * ((g.H + 3) / 4);
+    const dim3 gridp(nbp, nitems);
     switch (mode) {
-        case FP_MODE_FP: hipLaunchKernelGGL((k_fp<FP_MODE_FP, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
-        case FP_MODE_SART: hipLaunchKernelGGL((k_fp<FP_MODE_SART, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
+        case FP_MODE_FP: hipLaunchKernelGGL((k_fp_plane<FP_MODE_FP>), gridp, dim3(256), 0, s, items, g, factor); break;
+        case FP_MODE_SART: hipLaunchKernelGGL((k_fp_plane<FP_MODE_SART>), gridp, dim3(256), 0, s, items, g, factor); break;
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s) {
+    const dim3 grid((g.nx + 31) / 32, (g.nz + 31) / 32, g.ny * nitems);
+    hipLaunchKernelGGL(k_transpose_xz, grid, dim3(256), 0, s, items, g);
     return hipGetLastError();
 }
 

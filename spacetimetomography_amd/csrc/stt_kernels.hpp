@@ -13,6 +13,8 @@ enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2 };
 //   FP_MODE_SART    : corr = (F b - F Ax) / (F len + 1), raylen updated
 //   FP_MODE_SCATTER : contrib += factor*corr*w, wsum += w (wsum may be null)
 hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s);
+// Refreshes the transposed volume copy of every item whose view samples it.
+hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s);
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_bp_ray_apply(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_pd_dual(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s);
