@@ -201,6 +201,15 @@ int  stt_pd_extrapolate(stt_ctx* ctx, int frame);
  * timing may be NULL; when given, kernels are bracketed with HIP events. */
 int  stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph,
                  stt_timing* timing);
+/* Projector wave shape: rays per detector row of a 64-lane wave (64, 32, 16
+ * or 8; 256-thread blocks).  Process-wide setting. */
+int  stt_set_projector_tile(stt_ctx* ctx, int rays_per_row);
+/* Times every projector wave shape on the context's views (stand-alone
+ * forward projections into scratch buffers, no solver state changes) and
+ * keeps the fastest; *rays_per_row (may be NULL) receives the choice and
+ * ms_per_shape[4] (may be NULL) the average launch time of 64/32/16/8. */
+int  stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape);
+
 /* Blocks until the context stream is idle. */
 int  stt_synchronize(stt_ctx* ctx);
 

@@ -101,6 +101,8 @@ _SIGS = {
     "stt_pd_extrapolate": (C.c_int, [_P, C.c_int]),
     "stt_iterate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Timing)]),
     "stt_synchronize": (C.c_int, [_P]),
+    "stt_set_projector_tile": (C.c_int, [_P, C.c_int]),
+    "stt_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "stt_residual_norm": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "stt_power_iteration": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "stt_synth_phantom": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _F]),
@@ -294,3 +296,12 @@ class Context:
 
     def synchronize(self):
         check(lib.stt_synchronize(self.h))
+
+    def set_projector_tile(self, rays_per_row):
+        check(lib.stt_set_projector_tile(self.h, rays_per_row))
+
+    def autotune(self):
+        best = C.c_int()
+        ms = (C.c_double * 4)()
+        check(lib.stt_autotune(self.h, C.byref(best), ms))
+        return best.value, list(ms)

@@ -857,6 +857,68 @@ int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timin
     return STT_OK;
 }
 
+int stt_set_projector_tile(stt_ctx* ctx, int rays_per_row) {
+    CTX_CHECK(ctx);
+    if (rays_per_row != 64 && rays_per_row != 32 && rays_per_row != 16 && rays_per_row != 8)
+        return fail(STT_ERR_INVALID, "rays_per_row must be 64, 32, 16 or 8");
+    set_fp_tile(rays_per_row);
+    ctx->invalidate_graph();
+    return STT_OK;
+}
+
+int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
+    CTX_CHECK(ctx);
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    const int shapes[4] = {64, 32, 16, 8};
+    const size_t D = ctx->D;
+    float* buf = nullptr;
+    HIPCHK(hipMallocAsync((void**)&buf, 3 * D * sizeof(float), ctx->stream));
+    // one item per view of frame 0 (up to 16), outputs into scratch buffers
+    const int nv = std::min(ctx->P[0], 16);
+    std::vector<WorkItem> items;
+    for (int k = 0; k < nv; ++k) {
+        WorkItem it = ctx->make_item(0, k, true);
+        it.ax = buf;
+        it.raylen = buf + D;
+        it.weight = buf + 2 * D;
+        items.push_back(it);
+    }
+    rc = ctx->upload_scratch(items);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int old = fp_tile();
+    int best = old;
+    double best_ms = 1e30;
+    for (int si = 0; si < 4; ++si) {
+        set_fp_tile(shapes[si]);
+        for (int k = 0; k < nv; ++k) HIPCHK(launch_fp(FP_MODE_FP, ctx->items_scratch + k, 1, ctx->vg, 1.0f, ctx->stream));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        for (int rep = 0; rep < 2; ++rep)
+            for (int k = 0; k < nv; ++k) HIPCHK(launch_fp(FP_MODE_FP, ctx->items_scratch + k, 1, ctx->vg, 1.0f, ctx->stream));
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / (2.0 * nv);
+        if (ms_per_shape) ms_per_shape[si] = per;
+        if (per < best_ms) {
+            best_ms = per;
+            best = shapes[si];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_fp_tile(best);
+    if (best != old) ctx->invalidate_graph();
+    HIPCHK(hipFreeAsync(buf, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (rays_per_row) *rays_per_row = best;
+    return STT_OK;
+}
+
 int stt_synchronize(stt_ctx* ctx) {
     CTX_CHECK(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));

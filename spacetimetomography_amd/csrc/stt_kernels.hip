@@ -379,17 +379,17 @@ __device__ __forceinline__ float finalize_voxel(const WorkItem& it, const VolGeo
 
 // Sample generator of CastingRay (RayCaster.core.h:42-187): the entry sample,
 // then for each block segment the grid t = ceil((t1 - tnear)/sd)*sd + tnear,
-// t += sd while t < t_b, with the DDA advancing one tMax per segment.  Same
-// samples in the same order as cast_ray; it only lets the caller interleave
-// the rays of a wave by volume plane.
+// t += sd while t < t_b, the DDA advancing one tMax per finished segment.
+// Same samples, same order as cast_ray; only the interleaving across the
+// lanes of a wave changes.
 struct RayGen {
     float o[3], d[3], step[3];
     float tnear, tfar;
     float tMax[3], tDelta[3];
     float t1, tb, t;
-    float pos[3];
+    float pend[3];  // position of the pending (next) sample
     int segs;
-    bool in_seg, entry;
+    bool in_seg;
 };
 
 __device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGeom& g) {
@@ -405,7 +405,7 @@ __device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGe
     {
 #pragma clang fp contract(off)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) G.pos[c] = r.o[c] + r.d[c] * r.tnear;
+        for (int c = 0; c < 3; ++c) G.pend[c] = r.o[c] + r.d[c] * r.tnear;  // entry sample
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float dc = r.d[c];
@@ -413,11 +413,11 @@ __device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGe
                 G.tMax[c] = r.tfar;
                 G.tDelta[c] = 0.0f;
             } else if (dc < 0.0f) {
-                const int blk = ((int)ceilf(G.pos[c] - 0.5f) - 1) / dimv[c];
+                const int blk = ((int)ceilf(G.pend[c] - 0.5f) - 1) / dimv[c];
                 G.tMax[c] = ((float)(blk * dimv[c]) + 0.5f - r.o[c]) / dc;
                 G.tDelta[c] = (float)(-dimv[c]) / dc;
             } else {
-                const int blk = ((int)floorf(G.pos[c] - 0.5f)) / dimv[c];
+                const int blk = ((int)floorf(G.pend[c] - 0.5f)) / dimv[c];
                 G.tMax[c] = ((float)((blk + 1) * dimv[c]) + 0.5f - r.o[c]) / dc;
                 G.tDelta[c] = (float)dimv[c] / dc;
             }
@@ -427,26 +427,12 @@ __device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGe
     G.tb = G.t = 0.0f;
     G.segs = 0;
     G.in_seg = false;
-    G.entry = true;
 }
 
-// Next sample position into p; false when the ray has no more samples.
-__device__ __forceinline__ bool raygen_next(RayGen& G, float sd, float p[3]) {
-    if (G.entry) {
-        G.entry = false;
-        p[0] = G.pos[0]; p[1] = G.pos[1]; p[2] = G.pos[2];
-        return true;
-    }
+// Opens the next non-empty block segment (rare path); false when the ray ends.
+__device__ __forceinline__ bool raygen_next_segment(RayGen& G, float sd) {
     for (int guard = 0; guard < 10; ++guard) {
         if (G.in_seg) {
-            if (G.t < G.tb) {
-                p[0] = G.pos[0]; p[1] = G.pos[1]; p[2] = G.pos[2];
-                G.t += sd;
-                G.pos[0] += G.step[0];
-                G.pos[1] += G.step[1];
-                G.pos[2] += G.step[2];
-                return true;
-            }
             if (G.tMax[0] < G.tMax[1]) {
                 if (G.tMax[0] < G.tMax[2]) G.tMax[0] += G.tDelta[0];
                 else G.tMax[2] += G.tDelta[2];
@@ -465,83 +451,92 @@ __device__ __forceinline__ bool raygen_next(RayGen& G, float sd, float p[3]) {
 #pragma clang fp contract(off)
             G.t = ceilf((G.t1 - G.tnear) / sd) * sd + G.tnear;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) G.pos[c] = G.o[c] + G.d[c] * G.t;
+            for (int c = 0; c < 3; ++c) G.pend[c] = G.o[c] + G.d[c] * G.t;
         }
         G.in_seg = true;
         ++G.segs;
+        if (G.t < G.tb) return true;
     }
     return false;
 }
 
-__device__ __forceinline__ int wave_min_i32(int v) {
+__device__ __forceinline__ float wave_min_f32(float v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
     return v;
 }
-__device__ __forceinline__ int wave_max_i32(int v) {
+__device__ __forceinline__ float wave_max_f32(float v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
     return v;
 }
 
-// Plane-marching projector.  A wave holds 64 rays of one detector row; the
-// wave walks the volume plane by plane along the view's dominant axis D and a
-// lane takes its next sample only once the wave has reached that sample's
-// plane.  At any step the wave's taps then sit in one or two planes, close
-// together across the transverse axis, which the memory layout makes
-// contiguous: x-fastest for z-dominant views, the transposed z-fastest copy
-// for x-dominant views.  Each ray's samples, their order and its sums are
-// exactly those of CastingRay.
-template <int MODE, bool TRANSPOSED>
-__device__ __forceinline__ float march_ray(const VolView& vv, const VolGeom& g, const Ray& r, bool hit, int march,
+// Level-synchronous projector.  A wave holds 64 rays of a detector tile and
+// advances a depth level c along the view's dominant axis D by the smallest
+// per-sample step of its rays; a lane takes its next sample once the level
+// has reached it.  Rays of one tile are nearly parallel, so nearly every lane
+// samples on every step while the wave's taps stay within one or two planes,
+// close together across the transverse axis that the memory layout keeps
+// contiguous (x-fastest for z-dominant views, the transposed copy for
+// x-dominant views).  Samples, their order and each ray's sum are those of
+// CastingRay.
+template <int D, bool TRANSPOSED>
+__device__ __forceinline__ float march_ray(const VolView& vv, const VolGeom& g, const Ray& r, bool hit, bool down,
                                            int& nsamp) {
     RayGen G;
     raygen_init(G, r, g);
-    const int D = march & 3;
-    const int dir = (march & 4) ? -1 : 1;
-    const float bD = D == 0 ? g.bx : (D == 1 ? g.by : g.bz);
-    float p[3];
-    bool have = hit && raygen_next(G, g.sd, p);
-    auto plane_of = [&](const float q[3]) -> int {
-        const float c = D == 0 ? q[0] : (D == 1 ? q[1] : q[2]);
-        return (int)__builtin_amdgcn_fmed3f(c, 0.0f, bD);
-    };
-    int pl = have ? plane_of(p) : (dir > 0 ? 0x7fffffff : -0x7fffffff);
-    int cur = dir > 0 ? wave_min_i32(pl) : wave_max_i32(pl);
+    bool have = hit;
+    const float stepD = fabsf(G.step[D]);
+    const float s = wave_min_f32(have ? stepD : INFINITY);
+    float lvl = down ? wave_max_f32(have ? G.pend[D] : -INFINITY) : wave_min_f32(have ? G.pend[D] : INFINITY);
+    const float dl = down ? -s : s;
     float density = 0.0f;
     int n = 0;
-    const int limit = 2 * (g.nx + g.ny + g.nz) + g.max_samples + 16;
+    const int limit = 4 * (g.nx + g.ny + g.nz) + 2 * g.max_samples + 16;
     for (int it = 0; it < limit && __any(have); ++it) {
-        if (have && (dir > 0 ? pl <= cur : pl >= cur)) {
+        const bool take = have && (down ? G.pend[D] >= lvl : G.pend[D] <= lvl);
+        if (take) {
             Taps T;
-            if (TRANSPOSED) fetch_taps_t(vv, p[0], p[1], p[2], T);
-            else fetch_taps(vv, p[0], p[1], p[2], T);
-            density = fmaf(lerp_taps(T), g.sd, density);
+            if (TRANSPOSED) fetch_taps_t(vv, G.pend[0], G.pend[1], G.pend[2], T);
+            else fetch_taps(vv, G.pend[0], G.pend[1], G.pend[2], T);
+            bool ok = false;
+            if (G.in_seg) {
+                G.t += g.sd;
+                G.pend[0] += G.step[0];
+                G.pend[1] += G.step[1];
+                G.pend[2] += G.step[2];
+                ok = G.t < G.tb;
+            }
+            if (!ok) ok = raygen_next_segment(G, g.sd);
             ++n;
-            have = n < g.max_samples && raygen_next(G, g.sd, p);
-            if (have) pl = plane_of(p);
+            have = ok && n < g.max_samples;
+            density = fmaf(lerp_taps(T), g.sd, density);
         }
-        cur += dir;
+        lvl += dl;
     }
     nsamp = n;
-    // the entry sample is weighted like every other (RayCaster.core.h:60-61)
     return density;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fp_plane(const WorkItem* __restrict__ items, VolGeom g, float factor) {
+template <int MODE, int TX>
+__global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ items, VolGeom g, float factor) {
+    constexpr int TY = 256 / TX;
     int i, j;
-    const bool valid = tile_pixel<64, 4>(g, i, j);
+    const bool valid = tile_pixel<TX, TY>(g, i, j);
     const WorkItem& it = items[blockIdx.y];
     const ViewRec& v = *it.view;
     const int march = v.march;
+    const bool down = (march & 4) != 0;
     Ray r;
     produce_ray(v, g, (float)(valid ? i : 0), (float)(valid ? j : 0), r);
     const bool hit = valid && r.tnear < r.tfar;
     int ns = 0;
     float value;
-    if (MODE == FP_MODE_SART && (march & 8)) value = march_ray<MODE, true>(vol_view_t(it.volT, g), g, r, hit, march, ns);
-    else value = march_ray<MODE, false>(vol_view(it.vol, g), g, r, hit, march, ns);
+    const int D = march & 3;
+    if (MODE == FP_MODE_SART && (march & 8)) value = march_ray<0, true>(vol_view_t(it.volT, g), g, r, hit, down, ns);
+    else if (D == 0) value = march_ray<0, false>(vol_view(it.vol, g), g, r, hit, down, ns);
+    else if (D == 1) value = march_ray<1, false>(vol_view(it.vol, g), g, r, hit, down, ns);
+    else value = march_ray<2, false>(vol_view(it.vol, g), g, r, hit, down, ns);
     if (!valid) return;
     const int k = i + j * g.W;
     if (MODE == FP_MODE_FP) {
@@ -969,15 +964,33 @@ static inline dim3 vol_grid(const VolGeom& g, int nitems) {
     return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nitems);
 }
 
+// Tile width of the level-synchronous projector (rays per wave row); set by
+// stt_fp_tile() / the tuner.  64x4, 32x8, 16x16 or 8x32 rays per 256-thread block.
+static int g_fp_tile = 32;
+
+void set_fp_tile(int tx) { g_fp_tile = (tx == 64 || tx == 32 || tx == 16 || tx == 8) ? tx : 32; }
+int fp_tile() { return g_fp_tile; }
+
+template <int MODE>
+static hipError_t launch_fp_level(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
+    const int tx = g_fp_tile, ty = 256 / tx;
+    const dim3 grid(((g.W + tx - 1) / tx) * ((g.H + ty - 1) / ty), nitems);
+    switch (tx) {
+        case 64: hipLaunchKernelGGL((k_fp_level<MODE, 64>), grid, dim3(256), 0, s, items, g, factor); break;
+        case 16: hipLaunchKernelGGL((k_fp_level<MODE, 16>), grid, dim3(256), 0, s, items, g, factor); break;
+        case 8: hipLaunchKernelGGL((k_fp_level<MODE, 8>), grid, dim3(256), 0, s, items, g, factor); break;
+        default: hipLaunchKernelGGL((k_fp_level<MODE, 32>), grid, dim3(256), 0, s, items, g, factor); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
     constexpr int TX = 16, TY = 16;
     const int nb = ((g.W + TX - 1) / TX) * ((g.H + TY - 1) / TY);
     dim3 grid(nb, nitems);
-    const int nbp = ((g.W + 63) / 64) * ((g.H + 3) / 4);
-    const dim3 gridp(nbp, nitems);
     switch (mode) {
-        case FP_MODE_FP: hipLaunchKernelGGL((k_fp_plane<FP_MODE_FP>), gridp, dim3(256), 0, s, items, g, factor); break;
-        case FP_MODE_SART: hipLaunchKernelGGL((k_fp_plane<FP_MODE_SART>), gridp, dim3(256), 0, s, items, g, factor); break;
+        case FP_MODE_FP: return launch_fp_level<FP_MODE_FP>(items, nitems, g, factor, s);
+        case FP_MODE_SART: return launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
         default: return hipErrorInvalidValue;
     }

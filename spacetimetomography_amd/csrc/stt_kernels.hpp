@@ -12,6 +12,9 @@ enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2 };
 //   FP_MODE_FP      : ax, raylen (hit rays), weight
 //   FP_MODE_SART    : corr = (F b - F Ax) / (F len + 1), raylen updated
 //   FP_MODE_SCATTER : contrib += factor*corr*w, wsum += w (wsum may be null)
+// Rays per detector row of a projector wave (64, 32, 16 or 8).
+void set_fp_tile(int tx);
+int fp_tile();
 hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s);
 // Refreshes the transposed volume copy of every item whose view samples it.
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s);
