@@ -128,6 +128,8 @@ def main():
     t0 = time.time()
     g, p, ctx, angles, projections = setup(stt, cfg)
     log(f"setup done in {time.time() - t0:.1f}s")
+    tile, tile_ms = ctx.autotune()
+    log(f"autotune: projector tile {tile} (ms/view per tile: {[round(v, 4) for v in tile_ms]})")
     batched = args.form != "plain"
     use_graph = args.form == "batched-graph"
 
@@ -183,6 +185,7 @@ def main():
                          f"{P} golden-angle views/frame, {det[0]}x{det[1]} detector, voxel BP, clamped-cubic warp"),
             "form": args.form,
             "volume": list(vol), "frames": F, "views_per_frame": P, "detector": list(det),
+            "projector_tile": tile,
         },
         "roofline": {
             "bound": "hbm",

@@ -381,19 +381,71 @@ __device__ __forceinline__ float finalize_voxel(const WorkItem& it, const VolGeo
 // then for each block segment the grid t = ceil((t1 - tnear)/sd)*sd + tnear,
 // t += sd while t < t_b, the DDA advancing one tMax per finished segment.
 // Same samples, same order as cast_ray; only the interleaving across the
-// lanes of a wave changes.
+// lanes of a wave changes.  The first segment starts at t = tnear, on the
+// entry point itself, so when it is not empty the entry sample and that first
+// sample coincide and are taken once with weight 2 (RayCaster.core.h:60-61,
+// 137-145); a ray whose first segment is empty (a block boundary before the
+// entry, the pre-entry case) takes the entry sample on its own.
 struct RayGen {
     float o[3], d[3], step[3];
     float tnear, tfar;
     float tMax[3], tDelta[3];
     float t1, tb, t;
     float pend[3];  // position of the pending (next) sample
+    float w;        // weight (sample multiplicity) of the pending sample
     int segs;
-    bool in_seg;
 };
 
-__device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGeom& g) {
+// Opens the next non-empty block segment after t1; false when the ray ends.
+__device__ __forceinline__ bool raygen_open(RayGen& G, float sd) {
+#pragma nounroll
+    for (int guard = 0; guard < 8; ++guard) {
+        if (!(G.t1 < G.tfar) || G.segs >= 8) return false;
+        float tb = tmin(tmin(G.tMax[0], G.tMax[1]), G.tMax[2]);
+        if (tb > G.tfar) tb = G.tfar;
+        G.tb = tb;
+        {
+#pragma clang fp contract(off)
+            G.t = ceilf((G.t1 - G.tnear) / sd) * sd + G.tnear;
+            G.pend[0] = G.o[0] + G.d[0] * G.t;
+            G.pend[1] = G.o[1] + G.d[1] * G.t;
+            G.pend[2] = G.o[2] + G.d[2] * G.t;
+        }
+        ++G.segs;
+        if (G.t < G.tb) return true;
+        // empty segment: advance the DDA and try the next one
+        if (G.tMax[0] < G.tMax[1]) {
+            if (G.tMax[0] < G.tMax[2]) G.tMax[0] += G.tDelta[0];
+            else G.tMax[2] += G.tDelta[2];
+        } else {
+            if (G.tMax[1] < G.tMax[2]) G.tMax[1] += G.tDelta[1];
+            else G.tMax[2] += G.tDelta[2];
+        }
+        G.t1 = G.tb;
+    }
+    return false;
+}
+
+// Current segment exhausted: advance the DDA and open the next one.
+__device__ __forceinline__ bool raygen_next_segment(RayGen& G, float sd) {
+    if (G.tMax[0] < G.tMax[1]) {
+        if (G.tMax[0] < G.tMax[2]) G.tMax[0] += G.tDelta[0];
+        else G.tMax[2] += G.tDelta[2];
+    } else {
+        if (G.tMax[1] < G.tMax[2]) G.tMax[1] += G.tDelta[1];
+        else G.tMax[2] += G.tDelta[2];
+    }
+    G.t1 = G.tb;
+    return raygen_open(G, sd);
+}
+
+// Sets up the ray.  `entry_alone` is set when the entry sample has to be taken
+// on its own before the generator's pending sample; the return value says
+// whether the generator holds a pending sample.  A ray that misses the volume
+// takes no sample at all.
+__device__ __forceinline__ bool raygen_init(RayGen& G, const Ray& r, const VolGeom& g, bool hit, bool& entry_alone) {
     const int dimv[3] = {g.nx, g.ny, g.nz};
+    float pos1[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         G.o[c] = r.o[c];
@@ -405,7 +457,7 @@ __device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGe
     {
 #pragma clang fp contract(off)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) G.pend[c] = r.o[c] + r.d[c] * r.tnear;  // entry sample
+        for (int c = 0; c < 3; ++c) pos1[c] = r.o[c] + r.d[c] * r.tnear;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float dc = r.d[c];
@@ -413,51 +465,31 @@ __device__ __forceinline__ void raygen_init(RayGen& G, const Ray& r, const VolGe
                 G.tMax[c] = r.tfar;
                 G.tDelta[c] = 0.0f;
             } else if (dc < 0.0f) {
-                const int blk = ((int)ceilf(G.pend[c] - 0.5f) - 1) / dimv[c];
+                const int blk = ((int)ceilf(pos1[c] - 0.5f) - 1) / dimv[c];
                 G.tMax[c] = ((float)(blk * dimv[c]) + 0.5f - r.o[c]) / dc;
                 G.tDelta[c] = (float)(-dimv[c]) / dc;
             } else {
-                const int blk = ((int)floorf(G.pend[c] - 0.5f)) / dimv[c];
+                const int blk = ((int)floorf(pos1[c] - 0.5f)) / dimv[c];
                 G.tMax[c] = ((float)((blk + 1) * dimv[c]) + 0.5f - r.o[c]) / dc;
                 G.tDelta[c] = (float)dimv[c] / dc;
             }
         }
     }
     G.t1 = r.tnear;
-    G.tb = G.t = 0.0f;
     G.segs = 0;
-    G.in_seg = false;
-}
-
-// Opens the next non-empty block segment (rare path); false when the ray ends.
-__device__ __forceinline__ bool raygen_next_segment(RayGen& G, float sd) {
-    for (int guard = 0; guard < 10; ++guard) {
-        if (G.in_seg) {
-            if (G.tMax[0] < G.tMax[1]) {
-                if (G.tMax[0] < G.tMax[2]) G.tMax[0] += G.tDelta[0];
-                else G.tMax[2] += G.tDelta[2];
-            } else {
-                if (G.tMax[1] < G.tMax[2]) G.tMax[1] += G.tDelta[1];
-                else G.tMax[2] += G.tDelta[2];
-            }
-            G.t1 = G.tb;
-            G.in_seg = false;
-        }
-        if (!(G.t1 < G.tfar) || G.segs >= 8) return false;
-        float tb = tmin(tmin(G.tMax[0], G.tMax[1]), G.tMax[2]);
-        if (tb > G.tfar) tb = G.tfar;
-        G.tb = tb;
-        {
-#pragma clang fp contract(off)
-            G.t = ceilf((G.t1 - G.tnear) / sd) * sd + G.tnear;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) G.pend[c] = G.o[c] + G.d[c] * G.t;
-        }
-        G.in_seg = true;
-        ++G.segs;
-        if (G.t < G.tb) return true;
+    G.tb = G.t = 0.0f;
+    G.w = 1.0f;
+    entry_alone = false;
+    if (!hit) return false;
+    const bool open = raygen_open(G, g.sd);
+    // The first segment opens at t = ceil(0)*sd + tnear = tnear, the entry
+    // point; only when it is non-empty do the two samples coincide.
+    if (open && G.segs == 1) {
+        G.w = 2.0f;
+        return true;
     }
-    return false;
+    entry_alone = true;
+    return open;
 }
 
 __device__ __forceinline__ float wave_min_f32(float v) {
@@ -472,7 +504,7 @@ __device__ __forceinline__ float wave_max_f32(float v) {
 }
 
 // Level-synchronous projector.  A wave holds 64 rays of a detector tile and
-// advances a depth level c along the view's dominant axis D by the smallest
+// advances a depth level along the view's dominant axis D by the smallest
 // per-sample step of its rays; a lane takes its next sample once the level
 // has reached it.  Rays of one tile are nearly parallel, so nearly every lane
 // samples on every step while the wave's taps stay within one or two planes,
@@ -480,37 +512,46 @@ __device__ __forceinline__ float wave_max_f32(float v) {
 // contiguous (x-fastest for z-dominant views, the transposed copy for
 // x-dominant views).  Samples, their order and each ray's sum are those of
 // CastingRay.
-template <int D, bool TRANSPOSED>
-__device__ __forceinline__ float march_ray(const VolView& vv, const VolGeom& g, const Ray& r, bool hit, bool down,
-                                           int& nsamp) {
+template <int D, bool DOWN, bool TRANSPOSED>
+__device__ __forceinline__ float march_ray(const VolView& vv, const VolGeom& g, const Ray& r, bool hit, int& nsamp) {
     RayGen G;
-    raygen_init(G, r, g);
-    bool have = hit;
-    const float stepD = fabsf(G.step[D]);
-    const float s = wave_min_f32(have ? stepD : INFINITY);
-    float lvl = down ? wave_max_f32(have ? G.pend[D] : -INFINITY) : wave_min_f32(have ? G.pend[D] : INFINITY);
-    const float dl = down ? -s : s;
+    bool entry_alone;
+    bool have = raygen_init(G, r, g, hit, entry_alone);
     float density = 0.0f;
     int n = 0;
+    if (entry_alone) {  // pre-entry case: the entry sample first, on its own
+        float e0, e1, e2;
+        {
+#pragma clang fp contract(off)
+            e0 = r.o[0] + r.d[0] * r.tnear;
+            e1 = r.o[1] + r.d[1] * r.tnear;
+            e2 = r.o[2] + r.d[2] * r.tnear;
+        }
+        Taps T;
+        if (TRANSPOSED) fetch_taps_t(vv, e0, e1, e2, T);
+        else fetch_taps(vv, e0, e1, e2, T);
+        density = lerp_taps(T) * g.sd;
+        n = 1;
+    }
+    const float s = wave_min_f32(have ? fabsf(G.step[D]) : INFINITY);
+    float lvl = DOWN ? wave_max_f32(have ? G.pend[D] : -INFINITY) : wave_min_f32(have ? G.pend[D] : INFINITY);
+    const float dl = DOWN ? -s : s;
     const int limit = 4 * (g.nx + g.ny + g.nz) + 2 * g.max_samples + 16;
     for (int it = 0; it < limit && __any(have); ++it) {
-        const bool take = have && (down ? G.pend[D] >= lvl : G.pend[D] <= lvl);
-        if (take) {
+        if (have && (DOWN ? G.pend[D] >= lvl : G.pend[D] <= lvl)) {
             Taps T;
             if (TRANSPOSED) fetch_taps_t(vv, G.pend[0], G.pend[1], G.pend[2], T);
             else fetch_taps(vv, G.pend[0], G.pend[1], G.pend[2], T);
-            bool ok = false;
-            if (G.in_seg) {
-                G.t += g.sd;
-                G.pend[0] += G.step[0];
-                G.pend[1] += G.step[1];
-                G.pend[2] += G.step[2];
-                ok = G.t < G.tb;
-            }
-            if (!ok) ok = raygen_next_segment(G, g.sd);
-            ++n;
-            have = ok && n < g.max_samples;
-            density = fmaf(lerp_taps(T), g.sd, density);
+            const float w = G.w;
+            n += (int)w;
+            G.w = 1.0f;
+            G.t += g.sd;
+            G.pend[0] += G.step[0];
+            G.pend[1] += G.step[1];
+            G.pend[2] += G.step[2];
+            if (!(G.t < G.tb)) have = raygen_next_segment(G, g.sd);
+            have = have && n < g.max_samples;
+            density = fmaf(lerp_taps(T), w * g.sd, density);
         }
         lvl += dl;
     }
@@ -533,10 +574,15 @@ __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ i
     int ns = 0;
     float value;
     const int D = march & 3;
-    if (MODE == FP_MODE_SART && (march & 8)) value = march_ray<0, true>(vol_view_t(it.volT, g), g, r, hit, down, ns);
-    else if (D == 0) value = march_ray<0, false>(vol_view(it.vol, g), g, r, hit, down, ns);
-    else if (D == 1) value = march_ray<1, false>(vol_view(it.vol, g), g, r, hit, down, ns);
-    else value = march_ray<2, false>(vol_view(it.vol, g), g, r, hit, down, ns);
+    if (MODE == FP_MODE_SART && (march & 8)) {
+        const VolView vt = vol_view_t(it.volT, g);
+        value = down ? march_ray<0, true, true>(vt, g, r, hit, ns) : march_ray<0, false, true>(vt, g, r, hit, ns);
+    } else {
+        const VolView vv = vol_view(it.vol, g);
+        if (D == 0) value = down ? march_ray<0, true, false>(vv, g, r, hit, ns) : march_ray<0, false, false>(vv, g, r, hit, ns);
+        else if (D == 1) value = down ? march_ray<1, true, false>(vv, g, r, hit, ns) : march_ray<1, false, false>(vv, g, r, hit, ns);
+        else value = down ? march_ray<2, true, false>(vv, g, r, hit, ns) : march_ray<2, false, false>(vv, g, r, hit, ns);
+    }
     if (!valid) return;
     const int k = i + j * g.W;
     if (MODE == FP_MODE_FP) {
