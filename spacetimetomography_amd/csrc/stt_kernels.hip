@@ -660,98 +660,220 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 
 // Voxel-driven backprojection (ST_Tomography.cpp:687-731): project the voxel
 // centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
-// clamp [0, G_Max].  Atomic-free: each thread owns 4 consecutive x-voxels
-// (16-byte volume load/store, so enough bytes are in flight to stream HBM);
-// a 256-thread block covers 1024 voxels of one (y, z) slab row set, with
-// grid.z = z + nz*item.  The 4x4 detector window is read as four 16-byte rows
-// through a raw buffer.
-constexpr int kBpVox = 4;
+// clamp [0, G_Max].  Atomic-free, one thread per voxel column segment.
+//
+// A 256-thread block owns a 64 x 4 x 4 voxel tile (lane = x, wave = y, each
+// thread 4 consecutive z).  The tile is a box, so the detector positions of
+// its voxels lie inside the bounding box of its eight corners' projections;
+// the block stages that box of the correction image in LDS with row loads and
+// reads every voxel's 4x4 window from there: 16 taps per voxel from LDS
+// instead of four unaligned 16-byte global rows.  Stacking the tile along z
+// keeps the box small (about 1.3-1.6 detector pixels staged per voxel).
+//
+// Blocks whose box lies at least two pixels inside the [2, W-2) x [2, H-2)
+// window and fits in LDS take a fast path: every voxel is updated and its
+// window is inside the box (one pixel of margin absorbs the rounding of the
+// corner projections).  Every other block takes the general path: per-voxel
+// skip test with the reference's own arithmetic near the window edges, and a
+// window read from global memory when it is not inside the staged box.
+constexpr int kBpTileX = 64;
+constexpr int kBpTileY = 4;
+constexpr int kBpVoxZ = 4;  // voxels per thread along z
+constexpr int kBpLdsFloats = 4096;  // 16 KB staged detector footprint per block
+constexpr int kBpMaxCols = 192;     // three 64-lane column chunks
+constexpr int kBpMaxRows = 64;      // 16 rows per wave
 
-__device__ __forceinline__ float bp_update(const ViewRec& v, const VolGeom& g, const PdConst& pc,
-                                           __amdgpu_buffer_rsrc_t rs, float fx, float fy, float fz, float ex,
-                                           float ey, float ez, float xv, bool& touched) {
-    const float r = __builtin_amdgcn_rcpf(ez);
-    float pxf = fmaf(-ex, r, v.bp[12]);
-    float pyf = fmaf(-ey, r, v.bp[13]);
-    // Within 0.01 px of the detector window's edges the update/skip decision is
-    // taken with the reference's own arithmetic (rare, divergent).
-    const float hx = (float)(g.W - 2), hy = (float)(g.H - 2);
-    if (fabsf(pxf - 2.0f) < 0.01f || fabsf(pxf - hx) < 0.01f || fabsf(pyf - 2.0f) < 0.01f || fabsf(pyf - hy) < 0.01f) {
-        const float2 e = exact_detector_position(v, fx, fy, fz);
-        pxf = e.x;
-        pyf = e.y;
-    }
-    const int px = (int)pxf, py = (int)pyf;
-    touched = !(px < 2 || px >= g.W - 2 || py < 2 || py >= g.H - 2);
-    if (!touched) return xv;
-    const float dx = pxf - (float)px, dy = pyf - (float)py;
-    const unsigned off = 4u * (unsigned)((py - 1) * g.W + (px - 1));
-    const unsigned W4 = 4u * (unsigned)g.W;
+// Catmull-Rom weights (Horner form of cr_weights' polynomials).
+__device__ __forceinline__ void cr_weights_h(float t, float w[4]) {
+    const float t2 = t * t;
+    w[0] = t * fmaf(t, fmaf(-0.5f, t, 1.0f), -0.5f);
+    w[1] = fmaf(t2, fmaf(1.5f, t, -2.5f), 1.0f);
+    w[2] = t * fmaf(t, fmaf(-1.5f, t, 2.0f), 0.5f);
+    w[3] = t2 * fmaf(0.5f, t, -0.5f);
+}
+
+__device__ __forceinline__ float bp_bicubic(float dx, float dy, const float4 rows[4]) {
     float wy[4], wx[4];
-    cr_weights(dy, wy);
-    cr_weights(dx, wx);
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    cr_weights_h(dy, wy);
+    cr_weights_h(dx, wx);
+    float h[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const float4 row = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, b * W4, 0));
-        acc.x = fmaf(wy[b], row.x, acc.x);
-        acc.y = fmaf(wy[b], row.y, acc.y);
-        acc.z = fmaf(wy[b], row.z, acc.z);
-        acc.w = fmaf(wy[b], row.w, acc.w);
-    }
-    const float gv = wx[0] * acc.x + wx[1] * acc.y + wx[2] * acc.z + wx[3] * acc.w;
-    return __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xv), 0.0f, pc.gmax);
+    for (int b = 0; b < 4; ++b) h[b] = fmaf(wx[3], rows[b].w, fmaf(wx[2], rows[b].z, fmaf(wx[1], rows[b].y, wx[0] * rows[b].x)));
+    return fmaf(wy[3], h[3], fmaf(wy[2], h[2], fmaf(wy[1], h[1], wy[0] * h[0])));
 }
 
 __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ items, VolGeom g, PdConst pc) {
-    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * kBpVox;
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z % g.nz;
-    const WorkItem& it = items[blockIdx.z / g.nz];
-    if (x0 >= g.nx || y >= g.ny) return;
+    __shared__ float win[kBpLdsFloats];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nzt = (g.nz + kBpVoxZ - 1) / kBpVoxZ;
+    const int x0 = blockIdx.x * kBpTileX, y0 = blockIdx.y * kBpTileY;
+    const int z0 = (blockIdx.z % nzt) * kBpVoxZ;
+    const WorkItem& it = items[blockIdx.z / nzt];
     const ViewRec& v = *it.view;
-    const size_t q0 = (size_t)x0 + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)it.corr, (short)0, (int)g.det_bytes, kBufWord3);
     const float* A = v.bp;
-    const float fy = (float)y, fz = (float)z;
-    // detector-space row of x = x0, then one column of MV*V2M per voxel step
-    float ex = fmaf(A[0], (float)x0, fmaf(A[1], fy, fmaf(A[2], fz, A[3])));
-    float ey = fmaf(A[4], (float)x0, fmaf(A[5], fy, fmaf(A[6], fz, A[7])));
-    float ez = fmaf(A[8], (float)x0, fmaf(A[9], fy, fmaf(A[10], fz, A[11])));
-    gfloat* vol = gptr(it.vol);
-    const bool full = x0 + kBpVox <= g.nx && (g.nx & 3) == 0;
-    float xs[kBpVox];
-    if (full) {
-        const f32x4 q = *(const gf32x4*)(vol + q0);
-        xs[0] = q.x; xs[1] = q.y; xs[2] = q.z; xs[3] = q.w;
-    } else {
+    const int xe = min(x0 + kBpTileX, g.nx) - 1, ye = min(y0 + kBpTileY, g.ny) - 1, ze = min(z0 + kBpVoxZ, g.nz) - 1;
+
+    // Footprint: lanes 0..7 project one tile corner each.
+    int lx, ly, hx, hy;
+    {
+        const float cx = (float)((lane & 1) ? xe : x0);
+        const float cy = (float)((lane & 2) ? ye : y0);
+        const float cz = (float)((lane & 4) ? ze : z0);
+        const float ex = fmaf(A[0], cx, fmaf(A[1], cy, fmaf(A[2], cz, A[3])));
+        const float ey = fmaf(A[4], cx, fmaf(A[5], cy, fmaf(A[6], cz, A[7])));
+        const float ez = fmaf(A[8], cx, fmaf(A[9], cy, fmaf(A[10], cz, A[11])));
+        const float r = __builtin_amdgcn_rcpf(ez);
+        const float pxf = fminf(fmaxf(fmaf(-ex, r, v.bp[12]), -8.0f), (float)g.W + 8.0f);
+        const float pyf = fminf(fmaxf(fmaf(-ey, r, v.bp[13]), -8.0f), (float)g.H + 8.0f);
+        lx = (int)floorf(pxf);
+        ly = (int)floorf(pyf);
+        hx = lx;
+        hy = ly;
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i) xs[i] = (x0 + i < g.nx) ? vol[q0 + i] : 0.0f;
+        for (int m = 1; m <= 4; m <<= 1) {
+            lx = min(lx, __shfl_xor(lx, m, 64));
+            ly = min(ly, __shfl_xor(ly, m, 64));
+            hx = max(hx, __shfl_xor(hx, m, 64));
+            hy = max(hy, __shfl_xor(hy, m, 64));
+        }
+        lx = __builtin_amdgcn_readfirstlane(lx);
+        ly = __builtin_amdgcn_readfirstlane(ly);
+        hx = __builtin_amdgcn_readfirstlane(hx);
+        hy = __builtin_amdgcn_readfirstlane(hy);
     }
-    bool any = false;
+    // A voxel's window spans px-1..px+2; one more pixel of margin each side.
+    const int bx0 = max(lx - 2, 1), by0 = max(ly - 2, 1);
+    const int bx1 = min(hx + 3, g.W - 1), by1 = min(hy + 3, g.H - 1);
+    const int cols = bx1 - bx0 + 1, rows = by1 - by0 + 1;
+    const bool staged = cols > 0 && rows > 0 && cols <= kBpMaxCols && rows <= kBpMaxRows && cols * rows <= kBpLdsFloats;
+    const bool full_tile = x0 + kBpTileX <= g.nx && y0 + kBpTileY <= g.ny && z0 + kBpVoxZ <= g.nz;
+    const bool interior = staged && full_tile && lx >= 4 && ly >= 4 && hx <= g.W - 5 && hy <= g.H - 5;
+
+    // This thread's voxels are loaded before the footprint, so the two
+    // latencies overlap.
+    const int x = x0 + lane, y = y0 + wv;
+    const int nz_here = (x < g.nx && y < g.ny) ? min(kBpVoxZ, g.nz - z0) : 0;
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)it.vol, (short)0, (int)g.vol_bytes, kBufWord3);
+    const int vo = 4 * (x + g.nx * y);
+    const int plane4 = 4 * g.nx * g.ny;
+    float xs[kBpVoxZ];
 #pragma unroll
-    for (int i = 0; i < kBpVox; ++i) {
-        bool touched = false;
-        if (x0 + i < g.nx) xs[i] = bp_update(v, g, pc, rs, (float)(x0 + i), fy, fz, ex, ey, ez, xs[i], touched);
-        any |= touched;
-        ex += A[0];
-        ey += A[4];
-        ez += A[8];
+    for (int i = 0; i < kBpVoxZ; ++i)
+        xs[i] = i < nz_here ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rv, vo, (z0 + i) * plane4, 0)) : 0.0f;
+
+    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void*)it.corr, (short)0, (int)g.det_bytes, kBufWord3);
+    if (staged) {
+        // Rows wv, wv+4, ... of the box; up to 16 rows x 3 column chunks per
+        // wave, all loads in flight before the first LDS write.  Lanes past
+        // the box's last column re-read that column (every address stays
+        // inside the image) and store nothing.
+        int co[3];
+#pragma unroll
+        for (int h = 0; h < 3; ++h) co[h] = 4 * min(lane + 64 * h, cols - 1);
+        float vals[16][3];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int rr = wv + 4 * k;
+            if (rr < rows) {
+                const int so = 4 * ((by0 + rr) * g.W + bx0);
+#pragma unroll
+                for (int h = 0; h < 3; ++h)
+                    vals[k][h] = (64 * h < cols) ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rc, co[h], so, 0)) : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int rr = wv + 4 * k;
+            if (rr < rows) {
+#pragma unroll
+                for (int h = 0; h < 3; ++h)
+                    if (lane + 64 * h < cols) win[rr * cols + lane + 64 * h] = vals[k][h];
+            }
+        }
+    }
+    __syncthreads();
+    if (nz_here == 0) return;
+
+    const float fx = (float)x, fy = (float)y;
+    float ex = fmaf(A[0], fx, fmaf(A[1], fy, fmaf(A[2], (float)z0, A[3])));
+    float ey = fmaf(A[4], fx, fmaf(A[5], fy, fmaf(A[6], (float)z0, A[7])));
+    float ez = fmaf(A[8], fx, fmaf(A[9], fy, fmaf(A[10], (float)z0, A[11])));
+    bool any = false;
+    if (interior) {
+        const int base = -(by0 + 1) * cols - (bx0 + 1);
+#pragma unroll
+        for (int i = 0; i < kBpVoxZ; ++i) {
+            const float r = __builtin_amdgcn_rcpf(ez);
+            const float pxf = fmaf(-ex, r, v.bp[12]);
+            const float pyf = fmaf(-ey, r, v.bp[13]);
+            const int px = (int)pxf, py = (int)pyf;
+            const float* w0 = win + (py * cols + px + base);
+            float4 rw[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) rw[b] = make_float4(w0[b * cols], w0[b * cols + 1], w0[b * cols + 2], w0[b * cols + 3]);
+            const float gv = bp_bicubic(__builtin_amdgcn_fractf(pxf), __builtin_amdgcn_fractf(pyf), rw);
+            xs[i] = __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xs[i]), 0.0f, pc.gmax);
+            ex += A[2];
+            ey += A[6];
+            ez += A[10];
+        }
+        any = true;
+    } else {
+        const float hxw = (float)(g.W - 2), hyw = (float)(g.H - 2);
+#pragma unroll
+        for (int i = 0; i < kBpVoxZ; ++i) {
+            if (i < nz_here) {
+                const float fz = (float)(z0 + i);
+                const float r = __builtin_amdgcn_rcpf(ez);
+                float pxf = fmaf(-ex, r, v.bp[12]);
+                float pyf = fmaf(-ey, r, v.bp[13]);
+                // Within 0.01 px of the window's edges the update/skip decision
+                // is taken with the reference's own arithmetic (rare).
+                if (fabsf(pxf - 2.0f) < 0.01f || fabsf(pxf - hxw) < 0.01f || fabsf(pyf - 2.0f) < 0.01f ||
+                    fabsf(pyf - hyw) < 0.01f) {
+                    const float2 e = exact_detector_position(v, fx, fy, fz);
+                    pxf = e.x;
+                    pyf = e.y;
+                }
+                const int px = (int)pxf, py = (int)pyf;
+                if (!(px < 2 || px >= g.W - 2 || py < 2 || py >= g.H - 2)) {
+                    any = true;
+                    const int wx0 = px - 1 - bx0, wy0 = py - 1 - by0;
+                    float4 rw[4];
+                    if (staged && wx0 >= 0 && wx0 + 3 < cols && wy0 >= 0 && wy0 + 3 < rows) {
+                        const float* w0 = win + wy0 * cols + wx0;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            rw[b] = make_float4(w0[b * cols], w0[b * cols + 1], w0[b * cols + 2], w0[b * cols + 3]);
+                    } else {
+                        const int off = 4 * ((py - 1) * g.W + (px - 1));
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            rw[b] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rc, off, b * 4 * g.W, 0));
+                    }
+                    const float gv = bp_bicubic(pxf - (float)px, pyf - (float)py, rw);
+                    xs[i] = __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xs[i]), 0.0f, pc.gmax);
+                }
+            }
+            ex += A[2];
+            ey += A[6];
+            ez += A[10];
+        }
     }
     if (it.last) {
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i)
-            if (x0 + i < g.nx) xs[i] = finalize_voxel(it, g, pc, x0 + i, y, z, q0 + i, xs[i]);
+        for (int i = 0; i < kBpVoxZ; ++i)
+            if (i < nz_here) {
+                const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * (z0 + i));
+                xs[i] = finalize_voxel(it, g, pc, x, y, z0 + i, q, xs[i]);
+            }
     } else if (!any) {
         return;
     }
-    if (full) {
-        *(gf32x4*)(vol + q0) = f32x4{xs[0], xs[1], xs[2], xs[3]};
-    } else {
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i)
-            if (x0 + i < g.nx) vol[q0 + i] = xs[i];
-    }
+    for (int i = 0; i < kBpVoxZ; ++i)
+        if (i < nz_here) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, xs[i]), rv, vo, (z0 + i) * plane4, 0);
 }
 
 // Ray-BP normalisation (ST_Tomography.cpp:746-762): x += contrib/wsum with the
@@ -1050,7 +1172,8 @@ hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s) {
-    const dim3 grid((g.nx + 64 * kBpVox - 1) / (64 * kBpVox), (g.ny + 3) / 4, g.nz * nitems);
+    const dim3 grid((g.nx + kBpTileX - 1) / kBpTileX, (g.ny + kBpTileY - 1) / kBpTileY,
+                    ((g.nz + kBpVoxZ - 1) / kBpVoxZ) * nitems);
     hipLaunchKernelGGL(k_bp_voxel, grid, dim3(256), 0, s, items, g, pc);
     return hipGetLastError();
 }
