@@ -4,8 +4,8 @@
 Workload (default): space-time reconstruction of 8 frames of a 256^3 fp32
 volume, 45 golden-angle views per frame on a 512x512 detector (sid 600 mm,
 sdd 1000 mm, 1 mm pitch), primal-dual sigma = tau = 0.2, SART alpha = 0.3,
-lambda = 0.01, k = 0.1, e = 0.05, huber = 0.01, voxel-driven BP, clamped
-cubic warp.  Synthetic data: seeded ellipsoid phantom deformed by a smooth
+lambda = 0.01, k = 0.1, e = 0.05, huber = 0.01, STV spatio-temporal prior,
+voxel-driven BP, clamped cubic warp.  Synthetic data: seeded ellipsoid phantom deformed by a smooth
 sinusoidal field; projections are the forward projections of the phantom.
 
 One step = one ST iteration (a primal-dual pass over all frames: per frame
@@ -47,6 +47,7 @@ def setup(stt, cfg):
     p = stt.params_default()
     p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
     p.bp = stt.STT_BP_VOXEL
+    p.prior = stt.STT_PRIOR_STV
     p.interp = stt.STT_INTERP_CUBIC_CLAMPED
     ctx = stt.Context(g, p, [P] * F)
     angles = [stt.synth_angles(1, P, offset=P * f) for f in range(F)]
@@ -69,6 +70,39 @@ def setup(stt, cfg):
         ctx.set_volume(f, zero)
     ctx.pd_init()
     return g, p, ctx, angles, projections
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime (a plain
+    copy kernel of known size: reads and writes exactly mib MiB), so that a
+    counter pass can scale FETCH_SIZE / WRITE_SIZE against known bytes."""
+    hip = C.CDLL("libamdhip64.so.7")
+    n = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, n)
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, n, 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: {mib} MiB device-to-device")
+
+
+def load_traffic(config):
+    """HBM traffic per projector launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated against a copy kernel),
+    or None when no summary matches this configuration."""
+    path = os.path.join(ROOT, "profiles", f"traffic_k_fp_level_{config}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t
 
 
 def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
@@ -112,9 +146,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="st256", choices=sorted(CONFIGS))
     ap.add_argument("--form", default="batched-graph", choices=["batched-graph", "batched", "plain"])
-    ap.add_argument("--cpu-views", type=int, default=3, help="SART views in the CPU-baseline sample")
+    ap.add_argument("--cpu-views", type=int, default=0,
+                    help="SART views in the CPU-baseline sample (default: all views of one frame)")
     ap.add_argument("--cpu-lean", action="store_true", help="also time the lean CPU variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB first (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -125,6 +162,8 @@ def main():
     import spacetimetomography_amd as stt
 
     cfg = CONFIGS[args.config]
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     t0 = time.time()
     g, p, ctx, angles, projections = setup(stt, cfg)
     log(f"setup done in {time.time() - t0:.1f}s")
@@ -167,6 +206,7 @@ def main():
     iter_bytes = F * P * (12 * V + 12 * D) + F * 124 * V
     ms_per_step = elapsed / args.steps * 1e3
 
+    traffic = load_traffic(args.config) if batched else None
     out = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -182,22 +222,24 @@ def main():
         "data": "synthetic (seeded deformed ellipsoid phantom, projections = its forward projections)",
         "config": {
             "workload": (f"space-time PD+SART reconstruction {vol[0]}x{vol[1]}x{vol[2]} fp32 x {F} frames, "
-                         f"{P} golden-angle views/frame, {det[0]}x{det[1]} detector, voxel BP, clamped-cubic warp"),
+                         f"{P} golden-angle views/frame, {det[0]}x{det[1]} detector, STV prior, voxel BP, "
+                         f"clamped-cubic warp"),
             "form": args.form,
             "volume": list(vol), "frames": F, "views_per_frame": P, "detector": list(det),
             "projector_tile": tile,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_fp<SART> (forward projection + fused SART correction)",
+            "kernel": f"k_fp_level<SART,{tile}> (forward projection + fused SART correction, all frames' view k)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": None if traffic is None else traffic["bytes_per_launch"],
             "launch_avg_ms": fp_avg_s * 1e3,
             "bytes_per_launch": fp_bytes,
         },
+        "traffic_source": None if traffic is None else traffic["source"],
         "kernel_ms_per_step": {
             "fp": fp_ms / ksteps, "bp": bp_ms / ksteps, "pd_dual": dual_ms / ksteps, "pd_primal": primal_ms / ksteps,
         },
@@ -205,10 +247,11 @@ def main():
     }
     if not args.no_cpu_baseline:
         t1 = time.time()
-        out["cpu_baseline"] = cpu_baseline(g, p, cfg, angles, projections, args.cpu_views, faithful=True)
+        nv = args.cpu_views if args.cpu_views > 0 else cfg["P"]
+        out["cpu_baseline"] = cpu_baseline(g, p, cfg, angles, projections, nv, faithful=True)
         log(f"cpu baseline (faithful) {out['cpu_baseline']['value']:.5f} it/s in {time.time() - t1:.1f}s")
         if args.cpu_lean:
-            out["cpu_baseline_lean"] = cpu_baseline(g, p, cfg, angles, projections, args.cpu_views, faithful=False)
+            out["cpu_baseline_lean"] = cpu_baseline(g, p, cfg, angles, projections, nv, faithful=False)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -134,7 +134,9 @@ int  stt_set_projections(stt_ctx* ctx, int frame, const float* b); /* W*H*views[
 int  stt_set_volume_dev(stt_ctx* ctx, int frame, const float* vol_dev);
 int  stt_get_volume_dev(stt_ctx* ctx, int frame, float* vol_dev);
 
-/* Dual / auxiliary state readout (Y_tv 3 planes, Y_tp, Y_bc, X_bar, X_prev). */
+/* Dual / auxiliary state readout (X_bar, X_prev, Y_tp, Y_bc: one volume each;
+ * Y_tv: four planes, the three spatial components then STV's temporal
+ * component, which stays zero for the other priors). */
 #define STT_BUF_XBAR  0
 #define STT_BUF_XPREV 1
 #define STT_BUF_YTV   2

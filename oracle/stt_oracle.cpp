@@ -680,10 +680,16 @@ template <class R> static void pd_dual(OState<R>& s, int f, int warp_threads) {
                     y0 = MAXM(-s.lam, MINM(s.lam, y0));
                     y1 = MAXM(-s.lam, MINM(s.lam, y1));
                     y2 = MAXM(-s.lam, MINM(s.lam, y2));
-                } else {  // isotropic TV: projection onto the lambda L2 ball
+                } else if (s.p.prior == STT_PRIOR_TV) {  // isotropic TV: projection onto the lambda L2 ball
                     R nrm = std::sqrt(y0 * y0 + y1 * y1 + y2 * y2);
                     R sc = nrm > s.lam ? s.lam / nrm : R(1);
                     y0 *= sc; y1 *= sc; y2 *= sc;
+                } else {  // STV: 4-vector with the forward temporal difference, lambda L2 ball
+                    R y3 = ytv[q + 3 * V] + s.sigma * (xb1[q] - xb0[q]) * h;
+                    R nrm = std::sqrt(y0 * y0 + y1 * y1 + y2 * y2 + y3 * y3);
+                    R sc = nrm > s.lam ? s.lam / nrm : R(1);
+                    y0 *= sc; y1 *= sc; y2 *= sc; y3 *= sc;
+                    ytv[q + 3 * V] = y3;
                 }
                 ytv[q] = y0; ytv[q + V] = y1; ytv[q + 2 * V] = y2;
                 ytp[q] = ytp[q] * s.ktp / (s.ktp + R(2) * s.sigma);   // (:528)
@@ -715,6 +721,12 @@ template <class R> static void pd_primal(OState<R>& s, int f, int warp_threads) 
         }
     }
     const R* ytv = s.ytv[f].data();
+    // STV's temporal dual component of this frame and of the previous one
+    // (adjoint of the forward difference x[f+1] - x[f]: y[f-1] - y[f], with
+    // no y[-1])
+    const R* yt = ytv + 3 * V;
+    const R* ytm = f > 0 ? s.ytv[f - 1].data() + 3 * V : nullptr;
+    const bool stv = s.p.prior == STT_PRIOR_STV;
     const size_t sy = D[0], sz = (size_t)D[0] * D[1];
 #pragma omp parallel for schedule(static)
     for (int k = 1; k < D[2] - 1; ++k)
@@ -725,6 +737,7 @@ template <class R> static void pd_primal(OState<R>& s, int f, int warp_threads) 
                 // (:570-575)
                 R y1 = -(ytv[q] - ytv[q - 1] + ytv[V + q] - ytv[V + q - sy] + ytv[2 * V + q] - ytv[2 * V + q - sz]);
                 x[q] -= (s.tau * y1);
+                if (stv) x[q] -= s.tau * ((ytm ? ytm[q] : R(0)) - yt[q]);
                 x[q] = MAXM(MINM(s.gmax, x[q]), R(0));
             }
 }
@@ -774,7 +787,7 @@ template <class R> static o_ctx_base* make_state(const stt_geometry* g, const st
     s->xprev = s->x;
     s->ytp = s->x;
     s->ybc = s->x;
-    s->ytv.assign(F + 1, std::vector<R>(3 * s->V, R(0)));
+    s->ytv.assign(F + 1, std::vector<R>(4 * s->V, R(0)));  // 3 spatial planes + STV's temporal plane
     s->flow.assign(F, std::vector<R>(3 * s->V, R(0)));
     s->b.resize(F);
     for (int f = 0; f < F; ++f) s->b[f].assign((size_t)s->W * s->H * P[f], R(0));

@@ -234,7 +234,7 @@ class Context:
         check(lib.stt_set_projections(self.h, frame, fptr(np.ascontiguousarray(b, np.float32).reshape(-1))))
 
     def get_buffer(self, which, frame):
-        n = 3 * self.V if which == STT_BUF_YTV else self.V
+        n = 4 * self.V if which == STT_BUF_YTV else self.V
         out = np.empty(n, np.float32)
         check(lib.stt_get_buffer(self.h, which, frame, fptr(out)))
         return out

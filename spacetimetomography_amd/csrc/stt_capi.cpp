@@ -291,7 +291,7 @@ struct stt_ctx {
     int pd_init() {
         for (int f = 0; f <= F; ++f) {
             HIPCHK(hipMemcpyAsync(xbar[f], x[f], V * sizeof(float), hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemsetAsync(ytv[f], 0, 3 * V * sizeof(float), stream));
+            HIPCHK(hipMemsetAsync(ytv[f], 0, 4 * V * sizeof(float), stream));
             HIPCHK(hipMemsetAsync(ytp[f], 0, V * sizeof(float), stream));
             HIPCHK(hipMemsetAsync(ybc[f], 0, V * sizeof(float), stream));
             HIPCHK(hipMemsetAsync(xprev[f], 0, V * sizeof(float), stream));
@@ -426,7 +426,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
         if ((rc = c->alloc(&c->xT[f], c->VpadT))) return bail(rc);
         if ((rc = c->alloc(&c->xbar[f], c->V))) return bail(rc);
         if ((rc = c->alloc(&c->xprev[f], c->V))) return bail(rc);
-        if ((rc = c->alloc(&c->ytv[f], 3 * c->V))) return bail(rc);
+        if ((rc = c->alloc(&c->ytv[f], 4 * c->V))) return bail(rc);  // 3 spatial + temporal (STV) planes
         if ((rc = c->alloc(&c->ytp[f], c->V))) return bail(rc);
         if ((rc = c->alloc(&c->ybc[f], c->V))) return bail(rc);
     }
@@ -518,7 +518,7 @@ static int buffer_of(stt_ctx* ctx, int which, int frame, float** ptr, size_t* n)
     switch (which) {
         case STT_BUF_XBAR: *ptr = ctx->xbar[frame]; *n = ctx->V; break;
         case STT_BUF_XPREV: *ptr = ctx->xprev[frame]; *n = ctx->V; break;
-        case STT_BUF_YTV: *ptr = ctx->ytv[frame]; *n = 3 * ctx->V; break;
+        case STT_BUF_YTV: *ptr = ctx->ytv[frame]; *n = 4 * ctx->V; break;
         case STT_BUF_YTP: *ptr = ctx->ytp[frame]; *n = ctx->V; break;
         case STT_BUF_YBC: *ptr = ctx->ybc[frame]; *n = ctx->V; break;
         default: return fail(STT_ERR_INVALID, "unknown buffer id");

@@ -915,7 +915,8 @@ struct SamplePt {
     bool xedge;         // x+2 == nx: the fourth column is clamped to nx-1
 };
 
-__device__ __forceinline__ void sample_point(const VolGeom& g, float px, float py, float pz, bool cubic, SamplePt& s) {
+template <bool CUBIC>
+__device__ __forceinline__ void sample_point(const VolGeom& g, float px, float py, float pz, SamplePt& s) {
     const float d[3] = {(float)g.nx, (float)g.ny, (float)g.nz};
     const float p[3] = {px, py, pz};
     float gc[3];
@@ -926,7 +927,7 @@ __device__ __forceinline__ void sample_point(const VolGeom& g, float px, float p
     s.fy = gc[1] - (float)iy;
     s.fz = gc[2] - (float)iz;
     const unsigned ny4 = 4u * (unsigned)g.nx, nz4 = 4u * (unsigned)(g.nx * g.ny);
-    if (cubic) {
+    if (CUBIC) {
         cr_weights(s.fx, s.wx);
         cr_weights(s.fy, s.wy);
         cr_weights(s.fz, s.wz);
@@ -951,9 +952,9 @@ __device__ __forceinline__ void sample_point(const VolGeom& g, float px, float p
 // sample_image (warping.h:49-88) on channel plane `plane_off` (bytes) of an
 // image buffer: clamped Catmull-Rom bounded by the min/max of the 8 corners
 // of the cell (order x, y, z as CImg), or trilinear at the same point.
-__device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned plane_off, const SamplePt& s,
-                                            bool cubic) {
-    if (!cubic) {
+template <bool CUBIC>
+__device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned plane_off, const SamplePt& s) {
+    if (!CUBIC) {
         float2 r[2][2];
 #pragma unroll
         for (int kz = 0; kz < 2; ++kz)
@@ -993,8 +994,9 @@ __device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned 
 // tdt (= -dt of warp), then the image sample.  The first velocity sample sits
 // exactly on grid node clamp(i,1,n-2), where the clamped cubic returns the node
 // value, so it is read directly.  The three velocity planes share one buffer.
+template <bool CUBIC>
 __device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, const VolGeom& g, int i, int j, int k,
-                                            float tdt, bool cubic) {
+                                            float tdt) {
     const unsigned V = (unsigned)(g.nx * g.ny * g.nz);
     const __amdgpu_buffer_rsrc_t rs_img = __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, (int)(4u * V), kBufWord3);
     const __amdgpu_buffer_rsrc_t rs_vel =
@@ -1008,18 +1010,18 @@ __device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, 
 #pragma unroll
     for (int c = 0; c < 3; ++c) p2[c] = p[c] + tdt * v1[c];
     SamplePt s;
-    sample_point(g, p2[0], p2[1], p2[2], cubic, s);
+    sample_point<CUBIC>(g, p2[0], p2[1], p2[2], s);
     float v2[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v2[c] = sample_img(rs_vel, 4u * V * (unsigned)c, s, cubic);
+    for (int c = 0; c < 3; ++c) v2[c] = sample_img<CUBIC>(rs_vel, 4u * V * (unsigned)c, s);
     float q[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float p0 = p2[c] - tdt * v2[c];
         q[c] = p2[c] + 0.5f * (p[c] - p0);
     }
-    sample_point(g, q[0], q[1], q[2], cubic, s);
-    return sample_img(rs_img, 0u, s, cubic);
+    sample_point<CUBIC>(g, q[0], q[1], q[2], s);
+    return sample_img<CUBIC>(rs_img, 0u, s);
 }
 
 __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, const float* __restrict__ src,
@@ -1029,7 +1031,7 @@ __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, con
     const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    out[q] = warp_voxel(src, uvw, g, x, y, z, tdt, cubic != 0);
+    out[q] = cubic ? warp_voxel<true>(src, uvw, g, x, y, z, tdt) : warp_voxel<false>(src, uvw, g, x, y, z, tdt);
 }
 
 // Dual step of frame f (ST_Tomography.cpp:502-534); grid.z = z + nz*(f - f0).
@@ -1050,7 +1052,8 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
     gfloat* Ytp = gptr(B.ytp);
     gfloat* Ybc = gptr(B.ybc);
     // m_backward_warp = warp(u_f, X_bar[f+1], dt = -1): trace step +1 (:503)
-    const float bw = warp_voxel(B1.xbar, B.flow, g, x, y, z, 1.0f, pc.interp == 0);
+    const float bw = pc.interp == 0 ? warp_voxel<true>(B1.xbar, B.flow, g, x, y, z, 1.0f)
+                                    : warp_voxel<false>(B1.xbar, B.flow, g, x, y, z, 1.0f);
     const float xq = xb0[q];
     float ytp = Ytp[q] + pc.sigma * (xb1[q] - xq);  // (:504)
     float ybc = Ybc[q] + pc.sigma * (bw - xq);      // (:505)
@@ -1063,10 +1066,16 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
             y0 = tmax(-pc.lambda, tmin(pc.lambda, y0));
             y1 = tmax(-pc.lambda, tmin(pc.lambda, y1));
             y2 = tmax(-pc.lambda, tmin(pc.lambda, y2));
-        } else {  // isotropic TV: projection onto the lambda L2 ball
+        } else if (pc.prior == 1) {  // isotropic TV: projection onto the lambda L2 ball
             const float nrm = sqrtf(y0 * y0 + y1 * y1 + y2 * y2);
             const float sc = nrm > pc.lambda ? pc.lambda / nrm : 1.0f;
             y0 *= sc; y1 *= sc; y2 *= sc;
+        } else {  // STV: 4-vector with the forward temporal difference, lambda L2 ball
+            float y3 = Ytv[3 * V + q] + pc.sigma * (xb1[q] - xq) * h;
+            const float nrm = sqrtf(y0 * y0 + y1 * y1 + y2 * y2 + y3 * y3);
+            const float sc = nrm > pc.lambda ? pc.lambda / nrm : 1.0f;
+            y0 *= sc; y1 *= sc; y2 *= sc; y3 *= sc;
+            Ytv[3 * V + q] = y3;
         }
         Ytv[q] = y0;
         Ytv[V + q] = y1;
@@ -1099,7 +1108,8 @@ __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__
         xv = xv - pc.tau * (-Ybc[q]);
     } else {                // (:546-555): forward warp of Y_bc[f-1] with u_{f-1}, trace step -1
         const FrameBufs& Bm = fb[f - 1];
-        const float fw = warp_voxel(Bm.ybc, Bm.flow, g, x, y, z, -1.0f, pc.interp == 0);
+        const float fw = pc.interp == 0 ? warp_voxel<true>(Bm.ybc, Bm.flow, g, x, y, z, -1.0f)
+                                        : warp_voxel<false>(Bm.ybc, Bm.flow, g, x, y, z, -1.0f);
         xv = xv - pc.tau * (gptr(Bm.ytp)[q] - Ytp[q]);
         xv = xv - pc.tau * (fw - Ybc[q]);
     }
@@ -1107,6 +1117,10 @@ __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__
         const gfloat* t = gptr(B.ytv);
         const float y1 = -(t[q] - t[q - 1] + t[V + q] - t[V + q - sy] + t[2 * V + q] - t[2 * V + q - sz]);  // (:570-571)
         xv = xv - pc.tau * y1;
+        if (pc.prior == 2) {  // STV: adjoint of the forward temporal difference, y_t[f-1] - y_t[f]
+            const float ytm = f > 0 ? gptr(fb[f - 1].ytv)[3 * V + q] : 0.0f;
+            xv = xv - pc.tau * (ytm - t[3 * V + q]);
+        }
         xv = tmax(tmin(pc.gmax, xv), 0.0f);  // (:575)
     }
     X[q] = xv;

@@ -210,7 +210,7 @@ class OracleCtx:
         assert orc.o_ctx_set_projections(self.h, f, f32(np.ascontiguousarray(b, np.float32))) == 0
 
     def get_buffer(self, which, f):
-        n = 3 * self.V if which == 2 else self.V
+        n = 4 * self.V if which == 2 else self.V
         out = np.zeros(n, np.float32)
         assert orc.o_ctx_get_buffer(self.h, which, f, f32(out)) == 0
         return out

@@ -199,12 +199,18 @@ def test_sart_sweep_matches_oracle():
     ctx.close()
 
 
-@pytest.mark.parametrize("interp", [stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_INTERP_TRILINEAR])
-def test_pd_steps_match_oracle(interp):
-    print(f"[pd-steps] interp={interp}", flush=True)
+@pytest.mark.parametrize("interp,prior", [
+    (stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_PRIOR_ATV),
+    (stt.STT_INTERP_TRILINEAR, stt.STT_PRIOR_ATV),
+    (stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_PRIOR_TV),
+    (stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_PRIOR_STV),
+])
+def test_pd_steps_match_oracle(interp, prior):
+    print(f"[pd-steps] interp={interp} prior={prior}", flush=True)
     p = stt.params_default()
     p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01
     p.interp = interp
+    p.prior = prior
     p.roi_cut[:] = [2, 3, 1, 4, 2, 2, 5, 4]
     p.roi_jtop = 3
     vol = (24, 22, 26)
@@ -227,11 +233,15 @@ def test_pd_steps_match_oracle(interp):
     ctx.close()
 
 
-@pytest.mark.parametrize("mode", ["plain", "batched", "graph"])
-def test_st_iteration_matches_oracle(mode):
-    print(f"[st-iter] mode={mode} F=3 28^3 det 40x40", flush=True)
+@pytest.mark.parametrize("mode,prior", [
+    ("plain", stt.STT_PRIOR_ATV), ("batched", stt.STT_PRIOR_ATV), ("graph", stt.STT_PRIOR_ATV),
+    ("graph", stt.STT_PRIOR_STV),
+])
+def test_st_iteration_matches_oracle(mode, prior):
+    print(f"[st-iter] mode={mode} prior={prior} F=3 28^3 det 40x40", flush=True)
     p = stt.params_default()
     p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    p.prior = prior
     P = (6, 5, 7)
     ga = lambda f: list(stt.synth_angles(1, P[f], offset=sum(P[:f])))
     g, ctx, oc = _setup_pair((28, 28, 28), (40, 40), P, ga, p, F=3)

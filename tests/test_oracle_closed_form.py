@@ -190,7 +190,8 @@ def test_dual_step_closed_form():
     gx[:, :, :-1] = X0[:, :, 1:] - X0[:, :, :-1]
     gy[:, :-1, :] = X0[:, 1:, :] - X0[:, :-1, :]
     gz[:-1, :, :] = X0[1:, :, :] - X0[:-1, :, :]
-    ytv = oc.get_buffer(stt.STT_BUF_YTV, 0).reshape(3, nz, ny, nx)
+    ytv = oc.get_buffer(stt.STT_BUF_YTV, 0).reshape(4, nz, ny, nx)
+    assert np.all(ytv[3] == 0)  # STV's temporal plane is unused by ATV
     for c, gg in enumerate((gx, gy, gz)):
         np.testing.assert_allclose(ytv[c], np.where(inner, sig * h * gg, 0.0), atol=1e-6)
     ytp = oc.get_buffer(stt.STT_BUF_YTP, 0).reshape(nz, ny, nx)
@@ -215,7 +216,7 @@ def test_dual_prox_clamps():
     oc.pd_dual(0)
     inner = np.zeros((8, 8, 8), bool)
     inner[1:7, 1:7, 1:7] = True
-    ytv = oc.get_buffer(stt.STT_BUF_YTV, 0).reshape(3, 8, 8, 8)
+    ytv = oc.get_buffer(stt.STT_BUF_YTV, 0).reshape(4, 8, 8, 8)[:3]
     assert np.abs(ytv).max() <= 0.05 + 1e-7 and np.isclose(np.abs(ytv[:, inner]).max(), 0.05)
     ytp = oc.get_buffer(stt.STT_BUF_YTP, 0).reshape(8, 8, 8)
     assert np.all(ytp[inner] == 0)          # k = 0 removes the temporal dual
@@ -242,7 +243,7 @@ def test_primal_tv_term_is_minus_adjoint_of_forward_difference():
     x = np.full(V, 50.0, np.float32)
     oc.set_volume(0, x)
     oc.pd_init()
-    oc.set_buffer(stt.STT_BUF_YTV, 0, Y.reshape(-1))
+    oc.set_buffer(stt.STT_BUF_YTV, 0, np.concatenate([Y.reshape(-1), np.zeros(V, np.float32)]))
     oc.pd_primal(0)  # sigma = 0 duals: only the TV term moves x
     xn = oc.get_volume(0).reshape(nz, ny, nx).astype(np.float64)
     divY = np.where(inner, (xn - 50.0) / 0.5, 0.0)  # x -= tau * (-div Y)
@@ -255,6 +256,104 @@ def test_primal_tv_term_is_minus_adjoint_of_forward_difference():
     lhs = sum(float(np.sum(g[c][inner] * Y[c][inner])) for c in range(3))
     rhs = -float(np.sum(u * divY))
     assert abs(lhs - rhs) <= 1e-4 * max(1.0, abs(lhs))
+
+
+def _grad4(X0, X1):
+    """Forward differences of frame X0 along x, y, z and to the next frame."""
+    g = [np.zeros_like(X0) for _ in range(4)]
+    g[0][:, :, :-1] = X0[:, :, 1:] - X0[:, :, :-1]
+    g[1][:, :-1, :] = X0[:, 1:, :] - X0[:, :-1, :]
+    g[2][:-1, :, :] = X0[1:, :, :] - X0[:-1, :, :]
+    g[3] = X1 - X0
+    return g
+
+
+def test_stv_dual_projects_spacetime_gradient_on_l2_ball():
+    """From zero duals, STV's dual is sigma*h*(grad_xyz, d_t) of X_bar scaled
+    onto the lambda L2 ball of the 4-vector; ATV leaves the temporal plane at 0."""
+    dims = (9, 8, 10)
+    nz, ny, nx = dims[::-1]
+    V = nx * ny * nz
+    p = stt.params_default()
+    p.prior = stt.STT_PRIOR_STV
+    p.sigma, p.lambda_tv, p.k_tp, p.e_bc, p.huber = 0.8, 0.3, 0.0, 1e6, 0.5
+    oc = _pd_ctx(dims, p)
+    rng = np.random.default_rng(14)
+    x0, x1 = rng.random(V, dtype=np.float32), rng.random(V, dtype=np.float32)
+    oc.set_volume(0, x0)
+    oc.set_volume(1, x1)
+    oc.pd_init()
+    oc.pd_dual(0)
+    X0 = x0.reshape(nz, ny, nx).astype(np.float64)
+    X1 = x1.reshape(nz, ny, nx).astype(np.float64)
+    h = 1.0 / (1.0 + 0.8 * 0.5 / 0.3)
+    g = np.stack([0.8 * h * c for c in _grad4(X0, X1)])
+    nrm = np.sqrt((g ** 2).sum(axis=0))
+    expect = g * np.minimum(1.0, 0.3 / np.maximum(nrm, 1e-30))
+    inner = np.zeros((nz, ny, nx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    ytv = oc.get_buffer(stt.STT_BUF_YTV, 0).reshape(4, nz, ny, nx)
+    np.testing.assert_allclose(ytv, np.where(inner, expect, 0.0), atol=2e-6)
+    # both regimes occur: some 4-vectors inside the ball, some projected onto it
+    assert (nrm[inner] < 0.3).any() and (nrm[inner] > 0.3).any()
+    assert np.abs(np.sqrt((ytv[:, inner] ** 2).sum(axis=0)) - np.minimum(nrm[inner], 0.3)).max() < 1e-6
+
+
+def test_stv_reduces_to_isotropic_tv_on_a_static_sequence():
+    dims = (8, 9, 7)
+    nz, ny, nx = dims[::-1]
+    V = nx * ny * nz
+    rng = np.random.default_rng(15)
+    x0 = rng.random(V, dtype=np.float32)
+    out = {}
+    for prior in (stt.STT_PRIOR_TV, stt.STT_PRIOR_STV):
+        p = stt.params_default()
+        p.prior = prior
+        p.sigma, p.lambda_tv, p.k_tp, p.e_bc, p.huber = 2.0, 0.1, 0.0, 1e6, 0.0
+        oc = _pd_ctx(dims, p)
+        oc.set_volume(0, x0)
+        oc.set_volume(1, x0)
+        oc.pd_init()
+        oc.pd_dual(0)
+        out[prior] = oc.get_buffer(stt.STT_BUF_YTV, 0).reshape(4, -1)
+    np.testing.assert_array_equal(out[stt.STT_PRIOR_STV], out[stt.STT_PRIOR_TV])
+    assert np.abs(out[stt.STT_PRIOR_TV][:3]).max() > 0.05
+
+
+def test_stv_primal_temporal_term_is_adjoint_of_forward_time_difference():
+    """<D_t u, y> = <u, D_t^T y> with (D_t u)[f] = u[f+1] - u[f] (the padding
+    frame is constant), where D_t^T y is read off the primal step of every
+    frame: x[f] -= tau * (y[f-1] - y[f])."""
+    dims = (7, 8, 6)
+    nz, ny, nx = dims[::-1]
+    V = nx * ny * nz
+    F = 3
+    p = stt.params_default()
+    p.prior = stt.STT_PRIOR_STV
+    p.tau, p.sigma, p.k_tp, p.e_bc, p.g_max = 0.25, 0.0, 0.0, 0.0, 1e6
+    oc = _pd_ctx(dims, p, F=F)
+    rng = np.random.default_rng(16)
+    inner = np.zeros((nz, ny, nx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    for f in range(F + 1):
+        oc.set_volume(f, np.full(V, 40.0, np.float32))
+    oc.pd_init()
+    Yt = rng.standard_normal((F, nz, ny, nx)).astype(np.float32)
+    Yt[:, ~inner] = 0
+    for f in range(F):
+        oc.set_buffer(stt.STT_BUF_YTV, f, np.concatenate([np.zeros(3 * V, np.float32), Yt[f].reshape(-1)]))
+    adj = []
+    for f in range(F):
+        oc.pd_primal(f)
+        adj.append((40.0 - oc.get_volume(f).reshape(nz, ny, nx).astype(np.float64)) / 0.25)
+    u = rng.standard_normal((F, nz, ny, nx))
+    u[:, ~inner] = 0
+    Dtu = [(u[f + 1] if f + 1 < F else 0.0) - u[f] for f in range(F)]
+    lhs = sum(float(np.sum(Dtu[f] * Yt[f])) for f in range(F))
+    rhs = sum(float(np.sum(u[f] * adj[f])) for f in range(F))
+    assert abs(lhs - rhs) <= 1e-4 * max(1.0, abs(lhs))
+    # and the term is zero off the interior
+    assert all(np.abs(a[~inner]).max() == 0 for a in adj)
 
 
 def test_roi_mask_rose_cut():
