@@ -150,6 +150,8 @@ def main():
                     help="SART views in the CPU-baseline sample (default: all views of one frame)")
     ap.add_argument("--cpu-lean", action="store_true", help="also time the lean CPU variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--projector-tile", type=int, default=-1,
+                    help="projector shape (0 slab, 64/32/16/8 level-synchronous); default: autotune")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (counter calibration)")
     args = ap.parse_args()
@@ -167,8 +169,14 @@ def main():
     t0 = time.time()
     g, p, ctx, angles, projections = setup(stt, cfg)
     log(f"setup done in {time.time() - t0:.1f}s")
-    tile, tile_ms = ctx.autotune()
-    log(f"autotune: projector tile {tile} (ms/view per tile: {[round(v, 4) for v in tile_ms]})")
+    if args.projector_tile >= 0:
+        tile = args.projector_tile
+        ctx.set_projector_tile(tile)
+        log(f"projector tile {tile} (fixed)")
+    else:
+        tile, tile_ms = ctx.autotune()
+        log(f"autotune: projector tile {tile} (ms per batched SART launch, shapes 0/64/32/16/8: "
+            f"{[round(v, 4) for v in tile_ms]})")
     batched = args.form != "plain"
     use_graph = args.form == "batched-graph"
 
@@ -183,7 +191,7 @@ def main():
 
     # Per-kernel durations from HIP events on the context stream (eager launches).
     ksteps = max(1, min(args.steps, 3))
-    fp_ms = bp_ms = dual_ms = primal_ms = 0.0
+    fp_ms = bp_ms = dual_ms = primal_ms = aux_ms = 0.0
     fp_n = bp_n = 0
     for _ in range(ksteps):
         t = ctx.iterate(1, batched=batched, use_graph=False, timing=True)
@@ -193,6 +201,7 @@ def main():
         bp_n += t.bp_launches
         dual_ms += t.dual_ms
         primal_ms += t.primal_ms
+        aux_ms += t.aux_ms
     ctx.synchronize()
     log(f"kernel timing: {ksteps} eager iterations")
 
@@ -242,6 +251,7 @@ def main():
         "traffic_source": None if traffic is None else traffic["source"],
         "kernel_ms_per_step": {
             "fp": fp_ms / ksteps, "bp": bp_ms / ksteps, "pd_dual": dual_ms / ksteps, "pd_primal": primal_ms / ksteps,
+            "layout_copy": aux_ms / ksteps,
         },
         "iteration_hbm_frac": iter_bytes / (ms_per_step / 1e3) / HBM_PEAK,
     }

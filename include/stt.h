@@ -95,6 +95,7 @@ typedef struct stt_timing {
     double bp_ms;        int bp_launches;
     double dual_ms;      int dual_launches;
     double primal_ms;    int primal_launches;
+    double aux_ms;       int aux_launches;   /* layout copies feeding the projector */
 } stt_timing;
 
 typedef struct stt_ctx stt_ctx;
@@ -203,13 +204,16 @@ int  stt_pd_extrapolate(stt_ctx* ctx, int frame);
  * timing may be NULL; when given, kernels are bracketed with HIP events. */
 int  stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph,
                  stt_timing* timing);
-/* Projector wave shape: rays per detector row of a 64-lane wave (64, 32, 16
- * or 8; 256-thread blocks).  Process-wide setting. */
+/* Projector shape (process-wide): 0 = slab projector (16x16 rays per block,
+ * volume boxes staged in LDS; default); 64, 32, 16 or 8 = level-synchronous
+ * projector with that many rays per detector row of a 64-lane wave. */
 int  stt_set_projector_tile(stt_ctx* ctx, int rays_per_row);
-/* Times every projector wave shape on the context's views (stand-alone
- * forward projections into scratch buffers, no solver state changes) and
- * keeps the fastest; *rays_per_row (may be NULL) receives the choice and
- * ms_per_shape[4] (may be NULL) the average launch time of 64/32/16/8. */
+/* Times every projector shape on the launches a SART sweep issues (for the
+ * first view indices k: the layout copy and the projector over every frame's
+ * view k, corrections and ray lengths into scratch buffers, so no solver
+ * state changes) and keeps the fastest; *rays_per_row (may be NULL) receives
+ * the choice and ms_per_shape[5] (may be NULL) the time per view index of
+ * shapes 0/64/32/16/8. */
 int  stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape);
 
 /* Blocks until the context stream is idle. */

@@ -57,7 +57,8 @@ class Timing(C.Structure):
                 ("fp_ms", C.c_double), ("fp_launches", C.c_int),
                 ("bp_ms", C.c_double), ("bp_launches", C.c_int),
                 ("dual_ms", C.c_double), ("dual_launches", C.c_int),
-                ("primal_ms", C.c_double), ("primal_launches", C.c_int)]
+                ("primal_ms", C.c_double), ("primal_launches", C.c_int),
+                ("aux_ms", C.c_double), ("aux_launches", C.c_int)]
 
 
 if not os.path.exists(LIB_PATH):
@@ -302,6 +303,6 @@ class Context:
 
     def autotune(self):
         best = C.c_int()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * 5)()
         check(lib.stt_autotune(self.h, C.byref(best), ms))
         return best.value, list(ms)

@@ -240,6 +240,7 @@ struct stt_ctx {
                     case 2: t->bp_ms += ms; t->bp_launches++; break;
                     case 3: t->dual_ms += ms; t->dual_launches++; break;
                     case 4: t->primal_ms += ms; t->primal_launches++; break;
+                    case 5: t->aux_ms += ms; t->aux_launches++; break;
                     default: break;
                 }
             }
@@ -260,6 +261,7 @@ struct stt_ctx {
             for (int k = 0; k < maxP; ++k) {
                 const WorkItem* it = items_batch + batch_off[k];
                 HIPCHK(launch_transpose_xz(it, batch_cnt[k], vg, stream));
+                mark(5);
                 HIPCHK(launch_fp(FP_MODE_SART, it, batch_cnt[k], vg, pc.factor, stream));
                 mark(1);
                 int rc = launch_bp_for(it, batch_cnt[k], ray_bp);
@@ -276,6 +278,7 @@ struct stt_ctx {
                 for (int k = 0; k < P[f]; ++k) {
                     const WorkItem* it = items_plain + off + k;
                     HIPCHK(launch_transpose_xz(it, 1, vg, stream));
+                    mark(5);
                     HIPCHK(launch_fp(FP_MODE_SART, it, 1, vg, pc.factor, stream));
                     mark(1);
                     int rc = launch_bp_for(it, 1, ray_bp);
@@ -450,7 +453,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     if ((rc = c->alloc_t(&c->fb_dev, (size_t)F + 1))) return bail(rc);
     if ((rc = c->alloc_t(&c->items_plain, (size_t)c->total_views))) return bail(rc);
     if ((rc = c->alloc_t(&c->items_batch, (size_t)c->total_views))) return bail(rc);
-    c->scratch_cap = std::max(64, c->maxP + F);
+    c->scratch_cap = std::max(std::max(64, c->maxP + F), 4 * F);  // 4*F: the tuner's batched launches
     if ((rc = c->alloc_t(&c->items_scratch, (size_t)c->scratch_cap))) return bail(rc);
     c->views_host.resize(c->total_views);
     c->views_set.assign(F, 0);
@@ -859,8 +862,8 @@ int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timin
 
 int stt_set_projector_tile(stt_ctx* ctx, int rays_per_row) {
     CTX_CHECK(ctx);
-    if (rays_per_row != 64 && rays_per_row != 32 && rays_per_row != 16 && rays_per_row != 8)
-        return fail(STT_ERR_INVALID, "rays_per_row must be 64, 32, 16 or 8");
+    if (rays_per_row != 0 && rays_per_row != 64 && rays_per_row != 32 && rays_per_row != 16 && rays_per_row != 8)
+        return fail(STT_ERR_INVALID, "projector shape must be 0 (slab), 64, 32, 16 or 8");
     set_fp_tile(rays_per_row);
     ctx->invalidate_graph();
     return STT_OK;
@@ -870,39 +873,58 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     CTX_CHECK(ctx);
     int rc = ctx->ready_check();
     if (rc) return rc;
-    const int shapes[4] = {64, 32, 16, 8};
+    const int shapes[5] = {0, 64, 32, 16, 8};
     const size_t D = ctx->D;
-    float* buf = nullptr;
-    HIPCHK(hipMallocAsync((void**)&buf, 3 * D * sizeof(float), ctx->stream));
-    // one item per view of frame 0 (up to 16), outputs into scratch buffers
-    const int nv = std::min(ctx->P[0], 16);
+    // The launches a sweep issues: for view index k, the layout copy and the
+    // SART projector over every frame's view k (batched form), here for the
+    // first few k.  Corrections and ray lengths go to scratch buffers, so no
+    // solver state changes (the layout copy is rebuilt before every use).
+    const int nk = std::min(ctx->maxP, 4);
     std::vector<WorkItem> items;
-    for (int k = 0; k < nv; ++k) {
-        WorkItem it = ctx->make_item(0, k, true);
-        it.ax = buf;
-        it.raylen = buf + D;
-        it.weight = buf + 2 * D;
-        items.push_back(it);
+    std::vector<int> off, cnt;
+    for (int k = 0; k < nk; ++k) {
+        off.push_back((int)items.size());
+        for (int f = 0; f < ctx->F; ++f)
+            if (k < ctx->P[f]) items.push_back(ctx->make_item(f, k, false));
+        cnt.push_back((int)items.size() - off.back());
     }
+    float* buf = nullptr;
+    HIPCHK(hipMallocAsync((void**)&buf, 2 * items.size() * D * sizeof(float), ctx->stream));
+    for (size_t n = 0; n < items.size(); ++n) {
+        items[n].corr = buf + 2 * n * D;
+        items[n].raylen = buf + (2 * n + 1) * D;
+    }
+    HIPCHK(hipMemsetAsync(buf, 0, 2 * items.size() * D * sizeof(float), ctx->stream));
     rc = ctx->upload_scratch(items);
-    if (rc) return rc;
+    if (rc) {
+        (void)hipFreeAsync(buf, ctx->stream);
+        return rc;
+    }
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     const int old = fp_tile();
     int best = old;
     double best_ms = 1e30;
-    for (int si = 0; si < 4; ++si) {
+    auto sweep = [&]() -> hipError_t {
+        for (int k = 0; k < nk; ++k) {
+            const WorkItem* it = ctx->items_scratch + off[k];
+            hipError_t e = launch_transpose_xz(it, cnt[k], ctx->vg, ctx->stream);
+            if (e == hipSuccess) e = launch_fp(FP_MODE_SART, it, cnt[k], ctx->vg, ctx->pc.factor, ctx->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    for (int si = 0; si < 5; ++si) {
         set_fp_tile(shapes[si]);
-        for (int k = 0; k < nv; ++k) HIPCHK(launch_fp(FP_MODE_FP, ctx->items_scratch + k, 1, ctx->vg, 1.0f, ctx->stream));
+        HIPCHK(sweep());  // warm-up
         HIPCHK(hipEventRecord(e0, ctx->stream));
-        for (int rep = 0; rep < 2; ++rep)
-            for (int k = 0; k < nv; ++k) HIPCHK(launch_fp(FP_MODE_FP, ctx->items_scratch + k, 1, ctx->vg, 1.0f, ctx->stream));
+        for (int rep = 0; rep < 2; ++rep) HIPCHK(sweep());
         HIPCHK(hipEventRecord(e1, ctx->stream));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        const double per = ms / (2.0 * nv);
+        const double per = ms / (2.0 * nk);
         if (ms_per_shape) ms_per_shape[si] = per;
         if (per < best_ms) {
             best_ms = per;

@@ -606,6 +606,302 @@ __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ i
     }
 }
 
+// ---------------------------------------------------------------------------
+// Slab projector.  A 256-thread block owns a 16 x 16 detector tile (wave w =
+// tile rows 4w..4w+3) and marches it through the volume in slabs of up to
+// kSlabK planes along the view's dominant axis D.  All rays of a view leave
+// the same source point, so at a plane c the tile's rays lie between the
+// extreme slopes dA/dD, dB/dD of its rays on the two transverse axes; the
+// block stages the box of voxels those positions can touch (with one voxel of
+// margin and the +1 taps) from the x-fastest volume into LDS with row loads,
+// then every lane takes the samples of its ray whose base plane lies in the
+// slab and reads the 8 taps from LDS.  The sample sequence is CastingRay's
+// (the same generator as march_ray), only the tap source changes: a sample
+// whose taps are not inside the staged box (behind the slab -- the pre-entry
+// samples of grazing rays --, off the box by rounding, or a slab too wide for
+// LDS) reads them from global memory.  No layout copy of the volume is needed
+// for any view direction.
+constexpr int kSlabCap = 6144;  // floats of staged box per block (24 KB)
+constexpr int kSlabK = 23;      // sample planes per slab (box depth <= 24)
+
+struct SlabBox {
+    int lo[3], n[3];   // box origin and extent per volume axis (x, y, z)
+    int cbeg, cend;    // sample base planes of the slab: [cbeg, cend) (up) / (cend, cbeg] (down)
+    int px, pxy;       // LDS row and plane pitch (odd, so lanes on one x spread over the banks)
+    int base;          // LDS index of (lo), subtracted from tap indices
+    bool staged;
+};
+
+// Block-wide min/max of five values (wave shuffles, then one LDS round).
+__device__ __forceinline__ void block_minmax5(float v[5], const bool is_max[5], float* red) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        v[k] = is_max[k] ? wave_max_f32(v[k]) : wave_min_f32(v[k]);
+        if (lane == 0) red[wv * 5 + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        float a = red[k];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) a = is_max[k] ? fmaxf(a, red[w * 5 + k]) : fminf(a, red[w * 5 + k]);
+        v[k] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a)));
+    }
+}
+
+// Box of the slab whose sample base planes start at c0 (uniform arithmetic).
+template <int D, bool DOWN>
+__device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const float o[3], float sAmin, float sAmax,
+                                            float sBmin, float sBmax) {
+    constexpr int A = D == 0 ? 1 : 0;
+    constexpr int B = D == 2 ? 1 : 2;
+    const int dims[3] = {g.nx, g.ny, g.nz};
+    SlabBox bx;
+    int K = kSlabK;
+    int nA = 0, nB = 0, loA = 0, loB = 0;
+#pragma unroll 1
+    for (int tries = 0; tries < 2; ++tries) {
+        // positions of the slab's samples along D: [c0, c0 + K) going up,
+        // [c0 - K + 1, c0 + 1) going down
+        const float e0 = (float)(DOWN ? c0 + 1 : c0) - o[D];
+        const float e1 = (float)(DOWN ? c0 - K + 1 : c0 + K) - o[D];
+        const float a0 = fminf(fminf(e0 * sAmin, e0 * sAmax), fminf(e1 * sAmin, e1 * sAmax)) + o[A];
+        const float a1 = fmaxf(fmaxf(e0 * sAmin, e0 * sAmax), fmaxf(e1 * sAmin, e1 * sAmax)) + o[A];
+        const float b0 = fminf(fminf(e0 * sBmin, e0 * sBmax), fminf(e1 * sBmin, e1 * sBmax)) + o[B];
+        const float b1 = fmaxf(fmaxf(e0 * sBmin, e0 * sBmax), fmaxf(e1 * sBmin, e1 * sBmax)) + o[B];
+        const float lim = 1.0e6f;
+        loA = max((int)floorf(fminf(fmaxf(a0, -lim), lim)) - 1, 0);
+        loB = max((int)floorf(fminf(fmaxf(b0, -lim), lim)) - 1, 0);
+        const int hiA = min((int)floorf(fminf(fmaxf(a1, -lim), lim)) + 2, dims[A]);
+        const int hiB = min((int)floorf(fminf(fmaxf(b1, -lim), lim)) + 2, dims[B]);
+        nA = hiA - loA + 1;
+        nB = hiB - loB + 1;
+        if (nA <= 0 || nB <= 0 || (long)(nA | 1) * nB * (K + 1) <= kSlabCap) break;
+        K = max(1, kSlabCap / ((nA | 1) * nB) - 1);  // thinner slab: its box only shrinks
+    }
+    // box along D: base planes of the slab and their +1 taps
+    int dlo, dhi;
+    if (DOWN) {
+        dlo = max(c0 - K + 1, 0);
+        dhi = min(c0 + 1, dims[D]);
+        bx.cbeg = c0;
+        bx.cend = c0 - K;
+    } else {
+        dlo = max(c0, 0);
+        dhi = min(c0 + K, dims[D]);
+        bx.cbeg = c0;
+        bx.cend = c0 + K;
+    }
+    bx.lo[D] = dlo;
+    bx.n[D] = dhi - dlo + 1;
+    bx.lo[A] = loA;
+    bx.n[A] = nA;
+    bx.lo[B] = loB;
+    bx.n[B] = nB;
+    bx.px = bx.n[0] | 1;
+    bx.pxy = (bx.px * bx.n[1]) | 1;
+    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap;
+    bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
+    return bx;
+}
+
+// Cooperative load of the box (x-fastest rows) into LDS.  Wave w loads the
+// z planes w, w+4, ...; within a plane the lanes cover rw columns of 64/rw
+// rows per load, so a row's voxel offset is a per-lane constant plus a
+// wave-uniform plane/row offset (scalar registers).  Indices up to dims (the
+// +1 taps of the last voxel, weight 0) read the volume's zero tail or the
+// next row, never past the allocation.
+__device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, const SlabBox& bx, float* box) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nbx = bx.n[0], nby = bx.n[1], nbz = bx.n[2];
+    const int rw = nbx <= 16 ? 16 : (nbx <= 32 ? 32 : 64);  // lanes per row
+    const int rpw = 64 / rw;                                // rows per load
+    const int sub = lane / rw, col = lane % rw;
+    const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
+    for (int cx = 0; cx < nbx; cx += rw) {
+        const int xcol = cx + col;
+        const bool col_ok = xcol < nbx;
+        const unsigned voff = 4u * ((unsigned)(bx.lo[0] + min(xcol, nbx - 1)) + nx * (unsigned)sub);
+        const int loff = min(xcol, nbx - 1) + bx.px * sub;
+        for (int zz = wv; zz < nbz; zz += 4) {
+            const unsigned zoff = 4u * (nxy * (unsigned)(bx.lo[2] + zz) + nx * (unsigned)bx.lo[1]);
+            const int lz = bx.pxy * zz;
+            // up to 8 row groups of this plane in flight before the LDS writes
+            for (int y0 = 0; y0 < nby; y0 += 8 * rpw) {
+                float vals[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int yy = y0 + m * rpw;
+                    vals[m] = 0.0f;
+                    if (yy + sub < nby)  // rows of the box only: every address inside the padded volume
+                        vals[m] = __builtin_bit_cast(
+                            float, __builtin_amdgcn_raw_buffer_load_b32(vv.rsrc, voff, zoff + 4u * nx * (unsigned)yy, 0));
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int yy = y0 + m * rpw;
+                    if (yy + sub < nby && col_ok) box[loff + lz + bx.px * yy] = vals[m];
+                }
+            }
+        }
+    }
+}
+
+template <int D, bool DOWN>
+__device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
+                                          float* box, float* red) {
+    constexpr int A = D == 0 ? 1 : 0;
+    constexpr int B = D == 2 ? 1 : 2;
+    const VolView vv = vol_view(it.vol, g);
+    const float bnd[3] = {g.bx, g.by, g.bz};
+    RayGen G;
+    bool entry_alone;
+    bool have = raygen_init(G, r, g, hit, entry_alone);
+    float density = 0.0f;
+    int n = 0;
+    if (entry_alone) {  // pre-entry case: the entry sample first, on its own
+        float e0, e1, e2;
+        {
+#pragma clang fp contract(off)
+            e0 = r.o[0] + r.d[0] * r.tnear;
+            e1 = r.o[1] + r.d[1] * r.tnear;
+            e2 = r.o[2] + r.d[2] * r.tnear;
+        }
+        Taps T;
+        fetch_taps(vv, e0, e1, e2, T);
+        density = lerp_taps(T) * g.sd;
+        n = 1;
+    }
+    // Slopes of the rays that march with the slabs, and the first base plane.
+    const bool fwd = DOWN ? r.d[D] < 0.0f : r.d[D] > 0.0f;
+    float q[5];
+    const bool qmax[5] = {false, true, false, true, DOWN};
+    {
+        const float sa = r.d[A] / r.d[D], sb = r.d[B] / r.d[D];
+        const bool use = have && fwd;
+        q[0] = use ? sa : INFINITY;
+        q[1] = use ? sa : -INFINITY;
+        q[2] = use ? sb : INFINITY;
+        q[3] = use ? sb : -INFINITY;
+        const float pD = floorf(__builtin_amdgcn_fmed3f(G.pend[D], 0.0f, bnd[D]));
+        q[4] = have ? pD : (DOWN ? -INFINITY : INFINITY);
+    }
+    block_minmax5(q, qmax, red);
+    float o[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r.o[c])));
+    const bool slopes_ok = q[0] <= q[1] && q[2] <= q[3];
+    const bool any_ray = DOWN ? q[4] > -INFINITY : q[4] < INFINITY;  // some lane of the block samples
+    int c0 = (int)fminf(fmaxf(q[4], -1.0f), (float)g.nx + g.ny + g.nz);
+    const int dimD = D == 0 ? g.nx : (D == 1 ? g.ny : g.nz);
+    const float sdw = g.sd;
+#pragma unroll 1
+    for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
+        SlabBox bx = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3]);
+        if (!slopes_ok) bx.staged = false;
+        if (bx.staged) slab_stage(vv, g, bx, box);
+        __syncthreads();
+        // One wave-uniform loop per slab: each iteration every lane whose
+        // pending sample lies in the slab (or behind it) takes it.
+#pragma unroll 1
+        for (int it = 0; it <= g.max_samples; ++it) {
+            const float cx = __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx);
+            const float cy = __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by);
+            const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
+            const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+            const int ic[3] = {ix, iy, iz};
+            const int p = ic[D];
+            const bool take = have && (DOWN ? p > bx.cend : p < bx.cend);
+            if (__builtin_amdgcn_ballot_w64(take) == 0) break;
+            if (take) {
+                Taps T;
+                T.fx = __builtin_amdgcn_fractf(cx);
+                T.fy = __builtin_amdgcn_fractf(cy);
+                T.fz = __builtin_amdgcn_fractf(cz);
+                const bool in_slab = DOWN ? p <= bx.cbeg : p >= bx.cbeg;
+                const bool in_box = bx.staged && in_slab && (unsigned)(ic[A] - bx.lo[A]) < (unsigned)(bx.n[A] - 1) &&
+                                    (unsigned)(ic[B] - bx.lo[B]) < (unsigned)(bx.n[B] - 1);
+                if (__builtin_expect(in_box, 1)) {
+                    const int li = ix + __mul24(iy, bx.px) + __mul24(iz, bx.pxy) - bx.base;
+                    const float* b0 = box + li;
+                    T.r00 = make_float2(b0[0], b0[1]);
+                    T.r10 = make_float2(b0[bx.px], b0[bx.px + 1]);
+                    T.r01 = make_float2(b0[bx.pxy], b0[bx.pxy + 1]);
+                    T.r11 = make_float2(b0[bx.pxy + bx.px], b0[bx.pxy + bx.px + 1]);
+                } else {
+                    const unsigned off =
+                        ((unsigned)ix << 2) + __umul24((unsigned)iy, vv.nx4) + __umul24((unsigned)iz, vv.sxy4);
+                    T.r00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vv.rsrc, off, 0, 0));
+                    T.r10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vv.rsrc, off, vv.nx4, 0));
+                    T.r01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vv.rsrc, off, vv.sxy4, 0));
+                    T.r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vv.rsrc, off, vv.sxynx4, 0));
+                }
+                const float w = G.w;
+                n += (int)w;
+                G.w = 1.0f;
+                G.t += sdw;
+                G.pend[0] += G.step[0];
+                G.pend[1] += G.step[1];
+                G.pend[2] += G.step[2];
+                if (__builtin_expect(!(G.t < G.tb), 0)) have = raygen_next_segment(G, sdw);
+                have = have && n < g.max_samples;
+                density = fmaf(lerp_taps(T), w * sdw, density);
+            }
+        }
+        c0 = bx.cend;
+        if (!__syncthreads_or(have)) break;
+    }
+    nsamp = n;
+    return density;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fp_slab(const WorkItem* __restrict__ items, VolGeom g, float factor) {
+    __shared__ float box[kSlabCap];
+    __shared__ float red[20];
+    int i, j;
+    const bool valid = tile_pixel<16, 16>(g, i, j);
+    const WorkItem& it = items[blockIdx.y];
+    const ViewRec& v = *it.view;
+    const int march = v.march & 7;
+    Ray r;
+    produce_ray(v, g, (float)(valid ? i : 0), (float)(valid ? j : 0), r);
+    const bool hit = valid && r.tnear < r.tfar;
+    int ns = 0;
+    float value;
+    switch (march) {
+        case 0: value = slab_ray<0, false>(it, g, r, hit, ns, box, red); break;
+        case 4: value = slab_ray<0, true>(it, g, r, hit, ns, box, red); break;
+        case 1: value = slab_ray<1, false>(it, g, r, hit, ns, box, red); break;
+        case 5: value = slab_ray<1, true>(it, g, r, hit, ns, box, red); break;
+        case 6: value = slab_ray<2, true>(it, g, r, hit, ns, box, red); break;
+        default: value = slab_ray<2, false>(it, g, r, hit, ns, box, red); break;
+    }
+    if (!valid) return;
+    const int k = i + j * g.W;
+    if (MODE == FP_MODE_FP) {
+        if (hit && it.raylen) gptr(it.raylen)[k] = r.tfar - r.tnear;
+        if (it.ax) gptr(it.ax)[k] = hit ? value : 0.0f;
+        if (it.weight) gptr(it.weight)[k] = (float)ns;
+    } else {
+        // SART correction (ST_Tomography.cpp:667-676); pY is zero when read.
+#pragma clang fp contract(off)
+        float len;
+        gfloat* rl = gptr(it.raylen);
+        if (hit) {
+            len = r.tfar - r.tnear;
+            rl[k] = len;
+        } else {
+            len = rl[k];  // stale value of the last view whose ray k hit
+        }
+        const float num = factor * gptr(it.b)[k] - factor * (hit ? value : 0.0f);
+        const float den = factor * len + 1.0f;
+        gptr(it.corr)[k] = num / den;
+    }
+}
+
 // x <-> z transpose of the volumes of the x-dominant views in a work table
 // (index x + nx*(y + ny*z) -> z + nz*(y + ny*x)) through a 32x33 LDS tile.
 __global__ __launch_bounds__(256) void k_transpose_xz(const WorkItem* __restrict__ items, VolGeom g) {
@@ -1146,12 +1442,23 @@ static inline dim3 vol_grid(const VolGeom& g, int nitems) {
     return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nitems);
 }
 
-// Tile width of the level-synchronous projector (rays per wave row); set by
-// stt_fp_tile() / the tuner.  64x4, 32x8, 16x16 or 8x32 rays per 256-thread block.
-static int g_fp_tile = 32;
+// Projector shape, set by stt_set_projector_tile() / the tuner: 0 = slab
+// projector (16x16 rays per block, LDS-staged volume boxes); 64, 32, 16, 8 =
+// level-synchronous projector with that many rays per wave row (64x4, 32x8,
+// 16x16 or 8x32 rays per 256-thread block, transposed copy for x-dominant
+// views in the SART sweep).
+static int g_fp_tile = 0;
 
-void set_fp_tile(int tx) { g_fp_tile = (tx == 64 || tx == 32 || tx == 16 || tx == 8) ? tx : 32; }
+void set_fp_tile(int tx) { g_fp_tile = (tx == 64 || tx == 32 || tx == 16 || tx == 8) ? tx : 0; }
 int fp_tile() { return g_fp_tile; }
+bool fp_needs_transpose() { return g_fp_tile != 0; }
+
+template <int MODE>
+static hipError_t launch_fp_slab(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
+    const dim3 grid(((g.W + 15) / 16) * ((g.H + 15) / 16), nitems);
+    hipLaunchKernelGGL((k_fp_slab<MODE>), grid, dim3(256), 0, s, items, g, factor);
+    return hipGetLastError();
+}
 
 template <int MODE>
 static hipError_t launch_fp_level(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
@@ -1171,8 +1478,12 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
     const int nb = ((g.W + TX - 1) / TX) * ((g.H + TY - 1) / TY);
     dim3 grid(nb, nitems);
     switch (mode) {
-        case FP_MODE_FP: return launch_fp_level<FP_MODE_FP>(items, nitems, g, factor, s);
-        case FP_MODE_SART: return launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
+        case FP_MODE_FP:
+            return g_fp_tile == 0 ? launch_fp_slab<FP_MODE_FP>(items, nitems, g, factor, s)
+                                  : launch_fp_level<FP_MODE_FP>(items, nitems, g, factor, s);
+        case FP_MODE_SART:
+            return g_fp_tile == 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
+                                  : launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
         default: return hipErrorInvalidValue;
     }
@@ -1180,6 +1491,7 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s) {
+    if (!fp_needs_transpose()) return hipSuccess;
     const dim3 grid((g.nx + 31) / 32, (g.nz + 31) / 32, g.ny * nitems);
     hipLaunchKernelGGL(k_transpose_xz, grid, dim3(256), 0, s, items, g);
     return hipGetLastError();

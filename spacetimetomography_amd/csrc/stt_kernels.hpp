@@ -12,11 +12,14 @@ enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2 };
 //   FP_MODE_FP      : ax, raylen (hit rays), weight
 //   FP_MODE_SART    : corr = (F b - F Ax) / (F len + 1), raylen updated
 //   FP_MODE_SCATTER : contrib += factor*corr*w, wsum += w (wsum may be null)
-// Rays per detector row of a projector wave (64, 32, 16 or 8).
+// Projector shape: 0 = slab projector (LDS-staged boxes, default); 64, 32,
+// 16 or 8 = level-synchronous projector with that many rays per wave row.
 void set_fp_tile(int tx);
 int fp_tile();
+bool fp_needs_transpose();
 hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s);
-// Refreshes the transposed volume copy of every item whose view samples it.
+// Refreshes the transposed volume copy of every item whose view samples it
+// (level-synchronous projector only; a no-op for the slab projector).
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s);
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_bp_ray_apply(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
