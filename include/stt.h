@@ -179,8 +179,11 @@ int  stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n);
  * on the context's volume grid; host pointers; aabb = [0, dims]. */
 int  stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt,
               float* out, int interp);
-/* VJPs of the warp: given g = dL/d(out), accumulate dL/d(src) into gsrc and
- * dL/d(uvw) into guvw (either may be NULL).  interp must be TRILINEAR. */
+/* VJPs of the trilinear warp (stt_warp with STT_INTERP_TRILINEAR): given
+ * g = dL/d(out), accumulate dL/d(src) into gsrc and dL/d(uvw) into guvw
+ * (host arrays, either may be NULL).  The clamps of the sampling grid to
+ * [1, n-2] have zero derivative; the first velocity sample is the node value
+ * of u at clamp(i, 1, n-2). */
 int  stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt,
                   const float* g, float* gsrc, float* guvw);
 
@@ -219,11 +222,15 @@ int  stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape);
 /* Blocks until the context stream is idle. */
 int  stt_synchronize(stt_ctx* ctx);
 
-/* FP64 reductions on the device: sum of squares of the data residual
- * b - A x over all frames and views (one forward projection per view). */
+/* FP64 reductions on the device, fixed summation order.
+ * r2 = sum over frames f and views k of ||b_fk - A_fk x_f||^2 (one forward
+ * projection per view into scratch; no solver state changes). */
 int  stt_residual_norm(stt_ctx* ctx, double* r2);
-/* Power iteration for the operator norm of the PD coupling operator
- * (spatial forward difference stacked with the temporal difference). */
+/* Power iteration (iters steps from a fixed pseudo-random start) for the
+ * norm of the PD coupling operator K over the context's F frames:
+ * K v = (forward differences of each frame along x, y, z on the dual's
+ * region -- the grid interior inside the ROI cut --, v[f+1] - v[f] over the
+ * whole volume with v[F] = 0).  *L = sqrt(lambda_max(K^T K)) estimate. */
 int  stt_power_iteration(stt_ctx* ctx, int iters, double* L);
 
 /* ---- host-side synthetic inputs (no device needed) ----------------------- */

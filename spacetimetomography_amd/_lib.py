@@ -278,6 +278,26 @@ class Context:
                            fptr(np.ascontiguousarray(src, np.float32)), dt, fptr(out), interp))
         return out
 
+    def warp_vjp(self, uvw, src, dt, g, gsrc=None, guvw=None):
+        """Adjoint of the trilinear warp: returns (gsrc, guvw) with the VJP
+        added to the given accumulators (zeros when not given)."""
+        gsrc = np.zeros(self.V, np.float32) if gsrc is None else np.ascontiguousarray(gsrc, np.float32)
+        guvw = np.zeros(3 * self.V, np.float32) if guvw is None else np.ascontiguousarray(guvw, np.float32)
+        check(lib.stt_warp_vjp(self.h, fptr(np.ascontiguousarray(uvw, np.float32)),
+                               fptr(np.ascontiguousarray(src, np.float32)), dt,
+                               fptr(np.ascontiguousarray(g, np.float32)), fptr(gsrc), fptr(guvw)))
+        return gsrc, guvw
+
+    def residual_norm(self):
+        r2 = C.c_double()
+        check(lib.stt_residual_norm(self.h, C.byref(r2)))
+        return r2.value
+
+    def power_iteration(self, iters):
+        L = C.c_double()
+        check(lib.stt_power_iteration(self.h, iters, C.byref(L)))
+        return L.value
+
     def pd_init(self):
         check(lib.stt_pd_init(self.h))
 

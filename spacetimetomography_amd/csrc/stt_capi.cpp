@@ -764,8 +764,24 @@ int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* 
 int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, const float* g, float* gsrc,
                  float* guvw) {
     CTX_CHECK(ctx);
-    (void)uvw; (void)src; (void)dt; (void)g; (void)gsrc; (void)guvw;
-    return fail(STT_ERR_STATE, "stt_warp_vjp: not available in this build");
+    if (!uvw || !src || !g) return fail(STT_ERR_INVALID, "null argument");
+    if (!gsrc && !guvw) return STT_OK;
+    const size_t V = ctx->V;
+    float* buf = nullptr;
+    HIPCHK(hipMallocAsync((void**)&buf, 9 * V * sizeof(float), ctx->stream));
+    float *d_uvw = buf, *d_src = buf + 3 * V, *d_g = buf + 4 * V, *d_gsrc = buf + 5 * V, *d_guvw = buf + 6 * V;
+    HIPCHK(hipMemcpyAsync(d_uvw, uvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_src, src, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_g, g, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (gsrc) HIPCHK(hipMemcpyAsync(d_gsrc, gsrc, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (guvw) HIPCHK(hipMemcpyAsync(d_guvw, guvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_warp_vjp(d_uvw, d_src, d_g, gsrc ? d_gsrc : nullptr, guvw ? d_guvw : nullptr, ctx->vg, dt,
+                           ctx->stream));
+    if (gsrc) HIPCHK(hipMemcpyAsync(gsrc, d_gsrc, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (guvw) HIPCHK(hipMemcpyAsync(guvw, d_guvw, 3 * V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(buf, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
 }
 
 int stt_pd_init(stt_ctx* ctx) {
@@ -949,14 +965,90 @@ int stt_synchronize(stt_ctx* ctx) {
 
 int stt_residual_norm(stt_ctx* ctx, double* r2) {
     CTX_CHECK(ctx);
-    (void)r2;
-    return fail(STT_ERR_STATE, "stt_residual_norm: not available in this build");
+    if (!r2) return fail(STT_ERR_INVALID, "null argument");
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    const size_t D = ctx->D;
+    const int F = ctx->F;
+    // one projector launch per view index k over every frame's view k; the
+    // line integrals and ray lengths go to scratch (no solver state changes)
+    std::vector<WorkItem> items;
+    std::vector<int> off, cnt;
+    for (int k = 0; k < ctx->maxP; ++k) {
+        off.push_back((int)items.size());
+        for (int f = 0; f < F; ++f)
+            if (k < ctx->P[f]) items.push_back(ctx->make_item(f, k, false));
+        cnt.push_back((int)items.size() - off.back());
+    }
+    const int nitems = (int)items.size();
+    float* ax = nullptr;
+    double* partial = nullptr;
+    WorkItem* dev_items = nullptr;
+    HIPCHK(hipMallocAsync((void**)&ax, 2 * (size_t)F * D * sizeof(float), ctx->stream));
+    HIPCHK(hipMallocAsync((void**)&partial, ((size_t)nitems * kReductionBlocks + 1) * sizeof(double), ctx->stream));
+    HIPCHK(hipMallocAsync((void**)&dev_items, (size_t)nitems * sizeof(WorkItem), ctx->stream));
+    for (int k = 0; k < ctx->maxP; ++k)
+        for (int n = 0; n < cnt[k]; ++n) {
+            WorkItem& it = items[off[k] + n];
+            it.ax = ax + (size_t)n * D;
+            it.raylen = ax + ((size_t)F + n) * D;
+            it.weight = nullptr;
+        }
+    HIPCHK(hipMemcpyAsync(dev_items, items.data(), (size_t)nitems * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
+    for (int k = 0; k < ctx->maxP; ++k) {
+        HIPCHK(launch_fp(FP_MODE_FP, dev_items + off[k], cnt[k], ctx->vg, 1.0f, ctx->stream));
+        for (int n = 0; n < cnt[k]; ++n) {
+            const WorkItem& it = items[off[k] + n];
+            HIPCHK(launch_residual_partial(it.b, it.ax, (int)D, partial + (size_t)(off[k] + n) * kReductionBlocks,
+                                           ctx->stream));
+        }
+    }
+    double* out = partial + (size_t)nitems * kReductionBlocks;
+    HIPCHK(launch_sum_partials(partial, nitems * kReductionBlocks, out, ctx->stream));
+    HIPCHK(hipMemcpyAsync(r2, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(ax, ctx->stream));
+    HIPCHK(hipFreeAsync(partial, ctx->stream));
+    HIPCHK(hipFreeAsync(dev_items, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
 }
 
 int stt_power_iteration(stt_ctx* ctx, int iters, double* L) {
     CTX_CHECK(ctx);
-    (void)iters; (void)L;
-    return fail(STT_ERR_STATE, "stt_power_iteration: not available in this build");
+    if (!L || iters < 1) return fail(STT_ERR_INVALID, "need iters >= 1 and an output");
+    const int F = ctx->F;
+    const size_t V = ctx->V, n = (size_t)F * V;
+    float *v = nullptr, *y = nullptr;
+    double* red = nullptr;
+    HIPCHK(hipMallocAsync((void**)&v, 2 * n * sizeof(float), ctx->stream));  // v, then w = K^T K v
+    HIPCHK(hipMallocAsync((void**)&y, 4 * n * sizeof(float), ctx->stream));
+    HIPCHK(hipMallocAsync((void**)&red, (kReductionBlocks + 1) * sizeof(double), ctx->stream));
+    float* w = v + n;
+    double* nrm2 = red + kReductionBlocks;
+    auto norm2 = [&](const float* a) -> hipError_t {
+        hipError_t e = launch_dot_partial(a, a, n, red, ctx->stream);
+        if (e == hipSuccess) e = launch_sum_partials(red, kReductionBlocks, nrm2, ctx->stream);
+        return e;
+    };
+    HIPCHK(launch_fill_hash(v, n, 0x5EEDu, ctx->stream));
+    HIPCHK(norm2(v));
+    HIPCHK(launch_scale_by_inv_sqrt(v, n, nrm2, ctx->stream));
+    for (int it = 0; it < iters; ++it) {
+        HIPCHK(launch_pi_apply_k(v, y, F, ctx->vg, ctx->pc, ctx->stream));
+        HIPCHK(launch_pi_apply_kt(y, w, F, ctx->vg, ctx->stream));
+        HIPCHK(norm2(w));  // ||K^T K v||^2 with ||v|| = 1
+        HIPCHK(launch_scale_by_inv_sqrt(w, n, nrm2, ctx->stream));
+        std::swap(v, w);
+    }
+    double lam2 = 0.0;
+    HIPCHK(hipMemcpyAsync(&lam2, nrm2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFreeAsync(std::min(v, w), ctx->stream));
+    HIPCHK(hipFreeAsync(y, ctx->stream));
+    HIPCHK(hipFreeAsync(red, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *L = std::sqrt(std::sqrt(lam2));  // lambda_max(K^T K) = ||K^T K v||; ||K|| = sqrt of it
+    return STT_OK;
 }
 
 }  // extern "C"

@@ -1436,6 +1436,243 @@ __global__ __launch_bounds__(256) void k_extrapolate(const FrameBufs* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// FP64 reductions.  Fixed partition (grid-stride over kReductionBlocks
+// blocks, one partial per block, one final block), so every run sums in the
+// same order.
+
+__device__ __forceinline__ double block_sum_f64(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    return t;  // valid in thread 0
+}
+
+// partial[blockIdx.x] = sum over k of (b[k] - ax[k])^2 (ax = 0 for a missed ray).
+__global__ __launch_bounds__(256) void k_residual_partial(const float* __restrict__ b, const float* __restrict__ ax,
+                                                          int n, double* __restrict__ partial) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        const double r = (double)b[k] - (double)ax[k];
+        acc = fma(r, r, acc);
+    }
+    const double t = block_sum_f64(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// partial[blockIdx.x] = sum of a[k]*c[k] (a == c: squared norm).
+__global__ __launch_bounds__(256) void k_dot_partial(const float* __restrict__ a, const float* __restrict__ c, size_t n,
+                                                     double* __restrict__ partial) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+        acc = fma((double)a[k], (double)c[k], acc);
+    const double t = block_sum_f64(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// out[0] = sum of partial[0..n) in index order (one block).
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    __shared__ double red[4];
+    // each thread sums a contiguous run, so the order is fixed
+    const int per = (n + 255) / 256;
+    double acc = 0.0;
+    for (int k = threadIdx.x * per; k < min(n, (int)(threadIdx.x + 1) * per); ++k) acc += partial[k];
+    const double t = block_sum_f64(acc, red);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+// x *= s (s = 1/||x|| read from device memory as a double).
+__global__ __launch_bounds__(256) void k_scale_by_inv_sqrt(float* __restrict__ x, size_t n, const double* __restrict__ nrm2) {
+    const double v = nrm2[0];
+    const float s = v > 0.0 ? (float)(1.0 / sqrt(v)) : 0.0f;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) x[k] *= s;
+}
+
+// Deterministic start vector of the power iteration: x[k] in [-1, 1) from a
+// 32-bit integer hash of k.
+__global__ __launch_bounds__(256) void k_fill_hash(float* __restrict__ x, size_t n, unsigned seed) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        unsigned h = (unsigned)k * 0x9E3779B1u ^ seed;
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        x[k] = (float)(h >> 8) * (2.0f / 16777216.0f) - 1.0f;
+    }
+}
+
+// PD coupling operator of the power iteration: K v = (masked forward
+// differences along x, y, z on the dual's region (interior of the grid
+// inside the ROI cut, ST_Tomography.cpp:509-516), forward time difference
+// v[f+1] - v[f] over the whole volume with v[F] = 0).  v: F volumes, y: F
+// blocks of 4 planes.  grid.z = z + nz*f.
+__global__ __launch_bounds__(256) void k_pi_apply_k(const float* __restrict__ v, float* __restrict__ y, int F, VolGeom g,
+                                                    PdConst pc) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z % g.nz;
+    const int f = blockIdx.z / g.nz;
+    if (x >= g.nx || yy >= g.ny) return;
+    const size_t V = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t q = (size_t)x + sy * yy + sz * z;
+    const float* vf = v + V * f;
+    float* yf = y + 4 * V * f;
+    const float c = vf[q];
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if (x >= 1 && x < g.nx - 1 && yy >= 1 && yy < g.ny - 1 && z >= 1 && z < g.nz - 1 && !cut_out(g, pc, x, yy, z)) {
+        gx = vf[q + 1] - c;
+        gy = vf[q + sy] - c;
+        gz = vf[q + sz] - c;
+    }
+    yf[q] = gx;
+    yf[V + q] = gy;
+    yf[2 * V + q] = gz;
+    yf[3 * V + q] = (f + 1 < F ? v[V * (f + 1) + q] : 0.0f) - c;
+}
+
+// w = K^T y: minus the backward divergence of the spatial components plus
+// the adjoint of the time difference, y_t[f-1] - y_t[f] (no y_t[-1]).
+__global__ __launch_bounds__(256) void k_pi_apply_kt(const float* __restrict__ y, float* __restrict__ w, int F, VolGeom g) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z % g.nz;
+    const int f = blockIdx.z / g.nz;
+    if (x >= g.nx || yy >= g.ny) return;
+    const size_t V = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t q = (size_t)x + sy * yy + sz * z;
+    const float* yf = y + 4 * V * f;
+    float r = -(yf[q] + yf[V + q] + yf[2 * V + q]);
+    if (x > 0) r += yf[q - 1];
+    if (yy > 0) r += yf[V + q - sy];
+    if (z > 0) r += yf[2 * V + q - sz];
+    r += (f > 0 ? y[4 * V * (f - 1) + 3 * V + q] : 0.0f) - yf[3 * V + q];
+    w[V * f + q] = r;
+}
+
+// Adjoint of the trilinear warp (warping.h:100-145 with trilinear sampling):
+// for each output voxel, scatter g * d(out)/d(src) into gsrc and
+// g * d(out)/d(uvw) into guvw.  The trace is p2 = p + tdt*v1 (v1 = node value
+// of u at clamp(i,1,n-2)), v2 = S_u(p2), q = p2 + (p - (p2 - tdt*v2))/2,
+// out = S_src(q); the grid clamps to [1, n-2] have zero derivative outside.
+struct TriPt {
+    int ix, iy, iz;
+    float fx, fy, fz;
+    float dx, dy, dz;  // 1 inside the clamp range, 0 where the coordinate is clamped
+};
+
+__device__ __forceinline__ TriPt tri_point(const VolGeom& g, const float p[3]) {
+    const float d[3] = {(float)g.nx, (float)g.ny, (float)g.nz};
+    float gc[3], in[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float raw = (d[c] * p[c]) / d[c] - 0.5f;
+        gc[c] = __builtin_amdgcn_fmed3f(raw, 1.0f, d[c] - 2.0f);
+        in[c] = (raw > 1.0f && raw < d[c] - 2.0f) ? 1.0f : 0.0f;
+    }
+    TriPt t;
+    t.ix = (int)gc[0];
+    t.iy = (int)gc[1];
+    t.iz = (int)gc[2];
+    t.fx = gc[0] - (float)t.ix;
+    t.fy = gc[1] - (float)t.iy;
+    t.fz = gc[2] - (float)t.iz;
+    t.dx = in[0];
+    t.dy = in[1];
+    t.dz = in[2];
+    return t;
+}
+
+// Trilinear value and gradient (w.r.t. the sample position) of one plane.
+__device__ __forceinline__ float tri_eval(const gfloat* img, const VolGeom& g, const TriPt& t, float grad[3]) {
+    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t b = (size_t)t.ix + sy * t.iy + sz * t.iz;
+    const float c000 = img[b], c100 = img[b + 1], c010 = img[b + sy], c110 = img[b + sy + 1];
+    const float c001 = img[b + sz], c101 = img[b + sz + 1], c011 = img[b + sz + sy], c111 = img[b + sz + sy + 1];
+    const float gx = 1.0f - t.fx, gy = 1.0f - t.fy, gz = 1.0f - t.fz;
+    const float c00 = c000 * gx + c100 * t.fx, c10 = c010 * gx + c110 * t.fx;
+    const float c01 = c001 * gx + c101 * t.fx, c11 = c011 * gx + c111 * t.fx;
+    const float e0 = c00 * gy + c10 * t.fy, e1 = c01 * gy + c11 * t.fy;
+    grad[0] = t.dx * (gy * gz * (c100 - c000) + t.fy * gz * (c110 - c010) + gy * t.fz * (c101 - c001) +
+                      t.fy * t.fz * (c111 - c011));
+    grad[1] = t.dy * (gz * (c10 - c00) + t.fz * (c11 - c01));
+    grad[2] = t.dz * (e1 - e0);
+    return e0 * gz + e1 * t.fz;
+}
+
+__device__ __forceinline__ void tri_scatter(gfloat* img, const VolGeom& g, const TriPt& t, float val) {
+    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t b = (size_t)t.ix + sy * t.iy + sz * t.iz;
+    const float gx = 1.0f - t.fx, gy = 1.0f - t.fy, gz = 1.0f - t.fz;
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b, val * gx * gy * gz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + 1, val * t.fx * gy * gz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + sy, val * gx * t.fy * gz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + sy + 1, val * t.fx * t.fy * gz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + sz, val * gx * gy * t.fz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + sz + 1, val * t.fx * gy * t.fz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + sz + sy, val * gx * t.fy * t.fz);
+    __builtin_amdgcn_global_atomic_fadd_f32(img + b + sz + sy + 1, val * t.fx * t.fy * t.fz);
+}
+
+__global__ __launch_bounds__(256) void k_warp_vjp(const float* __restrict__ uvw, const float* __restrict__ src,
+                                                  const float* __restrict__ gout, float* __restrict__ gsrc,
+                                                  float* __restrict__ guvw, VolGeom g, float tdt) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const size_t V = (size_t)g.nx * g.ny * g.nz;
+    const size_t o = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
+    const float go = gptr(gout)[o];
+    if (go == 0.0f) return;
+    const gfloat* u = gptr(uvw);
+    const int ci = min(max(i, 1), g.nx - 2), cj = min(max(j, 1), g.ny - 2), ck = min(max(k, 1), g.nz - 2);
+    const size_t nq = (size_t)ci + (size_t)g.nx * ((size_t)cj + (size_t)g.ny * ck);
+    const float p[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+    const float v1[3] = {u[nq], u[V + nq], u[2 * V + nq]};
+    float p2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p2[c] = p[c] + tdt * v1[c];
+    const TriPt t2 = tri_point(g, p2);
+    float v2[3], J[3][3];  // J[c][d] = d v2_c / d p2_d
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v2[c] = tri_eval(u + V * c, g, t2, J[c]);
+    float q[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float p0 = p2[c] - tdt * v2[c];
+        q[c] = p2[c] + 0.5f * (p[c] - p0);
+    }
+    const TriPt tq = tri_point(g, q);
+    float a[3];
+    (void)tri_eval(gptr(src), g, tq, a);
+    if (gsrc) tri_scatter(gptr(gsrc), g, tq, go);
+    if (!guvw) return;
+    // dL/dq = go * grad S_src(q); q = (p + p2)/2 + tdt*v2/2
+    float dq[3], dv2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dq[c] = go * a[c];
+        dv2[c] = 0.5f * tdt * dq[c];
+    }
+    gfloat* gu = gptr(guvw);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tri_scatter(gu + V * c, g, t2, dv2[c]);
+    // dL/dp2 = dq/2 + J^T dv2; dL/dv1 = tdt * dL/dp2 (node value at clamp(i))
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float dp2 = 0.5f * dq[d] + J[0][d] * dv2[0] + J[1][d] * dv2[1] + J[2][d] * dv2[2];
+        __builtin_amdgcn_global_atomic_fadd_f32(gu + V * d + nq, tdt * dp2);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers.
 
 static inline dim3 vol_grid(const VolGeom& g, int nitems) {
@@ -1522,6 +1759,47 @@ hipError_t launch_pd_primal(const FrameBufs* fb, int f0, int nf, const VolGeom& 
 hipError_t launch_extrapolate(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_extrapolate, vol_grid(g, nf), dim3(256), 0, s, fb, f0, g, pc);
+    return hipGetLastError();
+}
+
+hipError_t launch_warp_vjp(const float* uvw, const float* src, const float* gout, float* gsrc, float* guvw,
+                           const VolGeom& g, float dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_warp_vjp, vol_grid(g, 1), dim3(256), 0, s, uvw, src, gout, gsrc, guvw, g, -dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_partial(const float* b, const float* ax, int n, double* partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_residual_partial, dim3(kReductionBlocks), dim3(256), 0, s, b, ax, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot_partial(const float* a, const float* c, size_t n, double* partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(kReductionBlocks), dim3(256), 0, s, a, c, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_by_inv_sqrt(float* x, size_t n, const double* nrm2, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_by_inv_sqrt, dim3(1024), dim3(256), 0, s, x, n, nrm2);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_hash(float* x, size_t n, unsigned seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_hash, dim3(1024), dim3(256), 0, s, x, n, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_pi_apply_k(const float* v, float* y, int F, const VolGeom& g, const PdConst& pc, hipStream_t s) {
+    hipLaunchKernelGGL(k_pi_apply_k, vol_grid(g, F), dim3(256), 0, s, v, y, F, g, pc);
+    return hipGetLastError();
+}
+
+hipError_t launch_pi_apply_kt(const float* y, float* w, int F, const VolGeom& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_pi_apply_kt, vol_grid(g, F), dim3(256), 0, s, y, w, F, g);
     return hipGetLastError();
 }
 

@@ -29,5 +29,18 @@ hipError_t launch_extrapolate(const FrameBufs* fb, int f0, int nf, const VolGeom
                               hipStream_t s);
 hipError_t launch_warp(const float* uvw, const float* src, float* out, const VolGeom& g, float dt, int cubic,
                        hipStream_t s);
+// Adjoint of the trilinear warp: gsrc += g dout/dsrc, guvw += g dout/duvw (atomics; either may be null).
+hipError_t launch_warp_vjp(const float* uvw, const float* src, const float* gout, float* gsrc, float* guvw,
+                           const VolGeom& g, float dt, hipStream_t s);
+// FP64 reductions: kReductionBlocks partials per call, then one ordered final sum.
+constexpr int kReductionBlocks = 256;
+hipError_t launch_residual_partial(const float* b, const float* ax, int n, double* partial, hipStream_t s);
+hipError_t launch_dot_partial(const float* a, const float* c, size_t n, double* partial, hipStream_t s);
+hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
+hipError_t launch_scale_by_inv_sqrt(float* x, size_t n, const double* nrm2, hipStream_t s);
+hipError_t launch_fill_hash(float* x, size_t n, unsigned seed, hipStream_t s);
+// Power-iteration operator K (masked spatial forward differences + time difference) and K^T.
+hipError_t launch_pi_apply_k(const float* v, float* y, int F, const VolGeom& g, const PdConst& pc, hipStream_t s);
+hipError_t launch_pi_apply_kt(const float* y, float* w, int F, const VolGeom& g, hipStream_t s);
 
 }  // namespace stt
