@@ -623,6 +623,7 @@ __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ i
 // for any view direction.
 constexpr int kSlabCap = 6144;  // floats of staged box per block (24 KB)
 constexpr int kSlabK = 23;      // sample planes per slab (box depth <= 24)
+constexpr int kStagePasses = 4; // staging covers x-y windows up to 4*256 elements
 
 struct SlabBox {
     int lo[3], n[3];   // box origin and extent per volume axis (x, y, z)
@@ -659,26 +660,30 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     const int dims[3] = {g.nx, g.ny, g.nz};
     SlabBox bx;
     int K = kSlabK;
-    int nA = 0, nB = 0, loA = 0, loB = 0;
-#pragma unroll 1
-    for (int tries = 0; tries < 2; ++tries) {
-        // positions of the slab's samples along D: [c0, c0 + K) going up,
-        // [c0 - K + 1, c0 + 1) going down
-        const float e0 = (float)(DOWN ? c0 + 1 : c0) - o[D];
-        const float e1 = (float)(DOWN ? c0 - K + 1 : c0 + K) - o[D];
-        const float a0 = fminf(fminf(e0 * sAmin, e0 * sAmax), fminf(e1 * sAmin, e1 * sAmax)) + o[A];
-        const float a1 = fmaxf(fmaxf(e0 * sAmin, e0 * sAmax), fmaxf(e1 * sAmin, e1 * sAmax)) + o[A];
-        const float b0 = fminf(fminf(e0 * sBmin, e0 * sBmax), fminf(e1 * sBmin, e1 * sBmax)) + o[B];
-        const float b1 = fmaxf(fmaxf(e0 * sBmin, e0 * sBmax), fmaxf(e1 * sBmin, e1 * sBmax)) + o[B];
-        const float lim = 1.0e6f;
-        loA = max((int)floorf(fminf(fmaxf(a0, -lim), lim)) - 1, 0);
-        loB = max((int)floorf(fminf(fmaxf(b0, -lim), lim)) - 1, 0);
-        const int hiA = min((int)floorf(fminf(fmaxf(a1, -lim), lim)) + 2, dims[A]);
-        const int hiB = min((int)floorf(fminf(fmaxf(b1, -lim), lim)) + 2, dims[B]);
-        nA = hiA - loA + 1;
-        nB = hiB - loB + 1;
-        if (nA <= 0 || nB <= 0 || (long)(nA | 1) * nB * (K + 1) <= kSlabCap) break;
-        K = max(1, kSlabCap / ((nA | 1) * nB) - 1);  // thinner slab: its box only shrinks
+    // positions of the slab's samples along D: [c0, c0 + K) going up,
+    // [c0 - K + 1, c0 + 1) going down; transverse extremes at the two ends
+    const float e0 = (float)(DOWN ? c0 + 1 : c0) - o[D];
+    const float e1 = (float)(DOWN ? c0 - K + 1 : c0 + K) - o[D];
+    const float a0 = fminf(fminf(e0 * sAmin, e0 * sAmax), fminf(e1 * sAmin, e1 * sAmax)) + o[A];
+    const float a1 = fmaxf(fmaxf(e0 * sAmin, e0 * sAmax), fmaxf(e1 * sAmin, e1 * sAmax)) + o[A];
+    const float b0 = fminf(fminf(e0 * sBmin, e0 * sBmax), fminf(e1 * sBmin, e1 * sBmax)) + o[B];
+    const float b1 = fmaxf(fmaxf(e0 * sBmin, e0 * sBmax), fmaxf(e1 * sBmin, e1 * sBmax)) + o[B];
+    const float lim = 1.0e6f;
+    const int loA = max((int)floorf(fminf(fmaxf(a0, -lim), lim)) - 1, 0);
+    const int loB = max((int)floorf(fminf(fmaxf(b0, -lim), lim)) - 1, 0);
+    const int hiA = min((int)floorf(fminf(fmaxf(a1, -lim), lim)) + 2, dims[A]);
+    const int hiB = min((int)floorf(fminf(fmaxf(b1, -lim), lim)) + 2, dims[B]);
+    const int nA = hiA - loA + 1;
+    const int nB = hiB - loB + 1;
+    if (nA > 0 && nB > 0) {
+        // thinner slab when the box would not fit; its true box only shrinks,
+        // so the extents above stay a valid (conservative) bound
+        const int per_plane = (nA | 1) * nB;
+        if (per_plane * (K + 1) > kSlabCap) {
+            K = (int)((float)kSlabCap * __builtin_amdgcn_rcpf((float)per_plane)) - 1;
+            while (K > 0 && per_plane * (K + 1) > kSlabCap) --K;
+            K = max(K, 1);
+        }
     }
     // box along D: base planes of the slab and their +1 taps
     int dlo, dhi;
@@ -701,49 +706,45 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     bx.n[B] = nB;
     bx.px = bx.n[0] | 1;
     bx.pxy = (bx.px * bx.n[1]) | 1;
-    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap;
+    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap &&
+                bx.n[0] * bx.n[1] <= 256 * kStagePasses;
     bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
     return bx;
 }
 
-// Cooperative load of the box (x-fastest rows) into LDS.  Wave w loads the
-// z planes w, w+4, ...; within a plane the lanes cover rw columns of 64/rw
-// rows per load, so a row's voxel offset is a per-lane constant plus a
-// wave-uniform plane/row offset (scalar registers).  Indices up to dims (the
+// Cooperative load of the box into LDS.  Each thread owns fixed positions
+// (column, row) of the box's x-y window -- element tid + 256*m of it -- and
+// walks the z planes, so a plane step moves only wave-uniform offsets.  Up
+// to 8 loads are in flight before their LDS writes.  Indices up to dims (the
 // +1 taps of the last voxel, weight 0) read the volume's zero tail or the
 // next row, never past the allocation.
 __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, const SlabBox& bx, float* box) {
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tid = threadIdx.x;
     const int nbx = bx.n[0], nby = bx.n[1], nbz = bx.n[2];
-    const int rw = nbx <= 16 ? 16 : (nbx <= 32 ? 32 : 64);  // lanes per row
-    const int rpw = 64 / rw;                                // rows per load
-    const int sub = lane / rw, col = lane % rw;
+    const int npl = nbx * nby;
+    const unsigned magic = (unsigned)(0xFFFFFFFFull / (unsigned)nbx) + 1u;  // e / nbx for e < 2^16
     const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
-    for (int cx = 0; cx < nbx; cx += rw) {
-        const int xcol = cx + col;
-        const bool col_ok = xcol < nbx;
-        const unsigned voff = 4u * ((unsigned)(bx.lo[0] + min(xcol, nbx - 1)) + nx * (unsigned)sub);
-        const int loff = min(xcol, nbx - 1) + bx.px * sub;
-        for (int zz = wv; zz < nbz; zz += 4) {
-            const unsigned zoff = 4u * (nxy * (unsigned)(bx.lo[2] + zz) + nx * (unsigned)bx.lo[1]);
-            const int lz = bx.pxy * zz;
-            // up to 8 row groups of this plane in flight before the LDS writes
-            for (int y0 = 0; y0 < nby; y0 += 8 * rpw) {
+    const unsigned zbase = 4u * nxy * (unsigned)bx.lo[2];
+#pragma unroll
+    for (int m = 0; m < kStagePasses; ++m) {
+        if (256 * m >= npl) break;  // uniform
+        const int e = tid + 256 * m;
+        const bool valid = e < npl;
+        const unsigned row = __umulhi((unsigned)e, magic);
+        const unsigned col = (unsigned)e - row * (unsigned)nbx;
+        const unsigned voff = 4u * ((unsigned)bx.lo[0] + col + nx * ((unsigned)bx.lo[1] + row));
+        const int loff = (int)col + bx.px * (int)row;
+        if (valid) {
+            for (int z0 = 0; z0 < nbz; z0 += 8) {
                 float vals[8];
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
-                    const int yy = y0 + m * rpw;
-                    vals[m] = 0.0f;
-                    if (yy + sub < nby)  // rows of the box only: every address inside the padded volume
-                        vals[m] = __builtin_bit_cast(
-                            float, __builtin_amdgcn_raw_buffer_load_b32(vv.rsrc, voff, zoff + 4u * nx * (unsigned)yy, 0));
-                }
+                for (int k = 0; k < 8; ++k)
+                    if (z0 + k < nbz)
+                        vals[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                                vv.rsrc, voff, zbase + 4u * nxy * (unsigned)(z0 + k), 0));
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
-                    const int yy = y0 + m * rpw;
-                    if (yy + sub < nby && col_ok) box[loff + lz + bx.px * yy] = vals[m];
-                }
+                for (int k = 0; k < 8; ++k)
+                    if (z0 + k < nbz) box[loff + bx.pxy * (z0 + k)] = vals[k];
             }
         }
     }
