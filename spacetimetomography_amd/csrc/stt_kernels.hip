@@ -621,7 +621,7 @@ __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ i
 // samples of grazing rays --, off the box by rounding, or a slab too wide for
 // LDS) reads them from global memory.  No layout copy of the volume is needed
 // for any view direction.
-constexpr int kSlabCap = 6144;  // floats of staged box per block (24 KB)
+constexpr int kSlabCap = 6144;  // floats of staged box per block (24 KB; 20 and 28 KB measured slower)
 constexpr int kSlabK = 23;      // sample planes per slab (box depth <= 24)
 constexpr int kStagePasses = 4; // staging covers x-y windows up to 4*256 elements
 
@@ -634,6 +634,7 @@ struct SlabBox {
 };
 
 // Block-wide min/max of five values (wave shuffles, then one LDS round).
+template <int NW>
 __device__ __forceinline__ void block_minmax5(float v[5], const bool is_max[5], float* red) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -646,7 +647,7 @@ __device__ __forceinline__ void block_minmax5(float v[5], const bool is_max[5], 
     for (int k = 0; k < 5; ++k) {
         float a = red[k];
 #pragma unroll
-        for (int w = 1; w < 4; ++w) a = is_max[k] ? fmaxf(a, red[w * 5 + k]) : fminf(a, red[w * 5 + k]);
+        for (int w = 1; w < NW; ++w) a = is_max[k] ? fmaxf(a, red[w * 5 + k]) : fminf(a, red[w * 5 + k]);
         v[k] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a)));
     }
 }
@@ -707,17 +708,18 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     bx.px = bx.n[0] | 1;
     bx.pxy = (bx.px * bx.n[1]) | 1;
     bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap &&
-                bx.n[0] * bx.n[1] <= 256 * kStagePasses;
+                bx.n[0] * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
     bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
     return bx;
 }
 
 // Cooperative load of the box into LDS.  Each thread owns fixed positions
-// (column, row) of the box's x-y window -- element tid + 256*m of it -- and
+// (column, row) of the box's x-y window -- element tid + NT*m of it -- and
 // walks the z planes, so a plane step moves only wave-uniform offsets.  Up
 // to 8 loads are in flight before their LDS writes.  Indices up to dims (the
 // +1 taps of the last voxel, weight 0) read the volume's zero tail or the
 // next row, never past the allocation.
+template <int NT>
 __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, const SlabBox& bx, float* box) {
     const int tid = threadIdx.x;
     const int nbx = bx.n[0], nby = bx.n[1], nbz = bx.n[2];
@@ -726,9 +728,9 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
     const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
     const unsigned zbase = 4u * nxy * (unsigned)bx.lo[2];
 #pragma unroll
-    for (int m = 0; m < kStagePasses; ++m) {
-        if (256 * m >= npl) break;  // uniform
-        const int e = tid + 256 * m;
+    for (int m = 0; m < kStagePasses * 256 / NT; ++m) {
+        if (NT * m >= npl) break;  // uniform
+        const int e = tid + NT * m;
         const bool valid = e < npl;
         const unsigned row = __umulhi((unsigned)e, magic);
         const unsigned col = (unsigned)e - row * (unsigned)nbx;
@@ -750,7 +752,7 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
     }
 }
 
-template <int D, bool DOWN>
+template <int D, bool DOWN, int NT>
 __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
                                           float* box, float* red) {
     constexpr int A = D == 0 ? 1 : 0;
@@ -789,7 +791,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         const float pD = floorf(__builtin_amdgcn_fmed3f(G.pend[D], 0.0f, bnd[D]));
         q[4] = have ? pD : (DOWN ? -INFINITY : INFINITY);
     }
-    block_minmax5(q, qmax, red);
+    block_minmax5<NT / 64>(q, qmax, red);
     float o[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r.o[c])));
@@ -802,7 +804,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
         SlabBox bx = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3]);
         if (!slopes_ok) bx.staged = false;
-        if (bx.staged) slab_stage(vv, g, bx, box);
+        if (bx.staged) slab_stage<NT>(vv, g, bx, box);
         __syncthreads();
         // One wave-uniform loop per slab: each iteration every lane whose
         // pending sample lies in the slab (or behind it) takes it.
@@ -858,12 +860,13 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     return density;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fp_slab(const WorkItem* __restrict__ items, VolGeom g, float factor) {
+template <int MODE, int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__ items, VolGeom g, float factor) {
+    constexpr int NT = TW * TH;
     __shared__ float box[kSlabCap];
-    __shared__ float red[20];
+    __shared__ float red[5 * NT / 64];
     int i, j;
-    const bool valid = tile_pixel<16, 16>(g, i, j);
+    const bool valid = tile_pixel<TW, TH>(g, i, j);
     const WorkItem& it = items[blockIdx.y];
     const ViewRec& v = *it.view;
     const int march = v.march & 7;
@@ -873,12 +876,12 @@ __global__ __launch_bounds__(256) void k_fp_slab(const WorkItem* __restrict__ it
     int ns = 0;
     float value;
     switch (march) {
-        case 0: value = slab_ray<0, false>(it, g, r, hit, ns, box, red); break;
-        case 4: value = slab_ray<0, true>(it, g, r, hit, ns, box, red); break;
-        case 1: value = slab_ray<1, false>(it, g, r, hit, ns, box, red); break;
-        case 5: value = slab_ray<1, true>(it, g, r, hit, ns, box, red); break;
-        case 6: value = slab_ray<2, true>(it, g, r, hit, ns, box, red); break;
-        default: value = slab_ray<2, false>(it, g, r, hit, ns, box, red); break;
+        case 0: value = slab_ray<0, false, NT>(it, g, r, hit, ns, box, red); break;
+        case 4: value = slab_ray<0, true, NT>(it, g, r, hit, ns, box, red); break;
+        case 1: value = slab_ray<1, false, NT>(it, g, r, hit, ns, box, red); break;
+        case 5: value = slab_ray<1, true, NT>(it, g, r, hit, ns, box, red); break;
+        case 6: value = slab_ray<2, true, NT>(it, g, r, hit, ns, box, red); break;
+        default: value = slab_ray<2, false, NT>(it, g, r, hit, ns, box, red); break;
     }
     if (!valid) return;
     const int k = i + j * g.W;
@@ -1694,7 +1697,7 @@ bool fp_needs_transpose() { return g_fp_tile != 0; }
 template <int MODE>
 static hipError_t launch_fp_slab(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
     const dim3 grid(((g.W + 15) / 16) * ((g.H + 15) / 16), nitems);
-    hipLaunchKernelGGL((k_fp_slab<MODE>), grid, dim3(256), 0, s, items, g, factor);
+    hipLaunchKernelGGL((k_fp_slab<MODE, 16, 16>), grid, dim3(256), 0, s, items, g, factor);
     return hipGetLastError();
 }
 

@@ -31,6 +31,9 @@ def _ctx(vol, det, spacing=(1, 1, 1), ds=1.0, sid=60.0, sdd=100.0, oxyz=(0, 0, 0
     return g, stt.Context(g, p, list(P))
 
 
+PROJECTOR_SHAPES = [0, 64, 32, 16, 8]  # slab projector, level-synchronous tiles
+
+
 @pytest.mark.parametrize("case", range(len(FP_CASES)))
 def test_fp_matches_oracle(case):
     vol, det, sp, ds, sid, sdd, oxyz, ang = FP_CASES[case]
@@ -41,15 +44,21 @@ def test_fp_matches_oracle(case):
     ctx.set_volume(0, x)
     ctx.set_views(0, [ang])
     rl0 = rng.random(det[0] * det[1], dtype=np.float32)  # stale values must survive on missed rays
-    ax, rl, w = ctx.forward_project(0, 0, raylen=rl0)
     q = O.quat_y(ang)
     oax, orl, ow = O.fp(g, q, x, raylen=rl0)
-    assert np.array_equal(w, ow), f"sample counts differ at {np.count_nonzero(w != ow)} rays"
-    assert np.array_equal(rl, orl), "ray lengths differ"
-    assert rel_l2(ax, oax) <= 1e-5
     ax64, _, _ = O.fp(g, q, x.astype(np.float64), f64mode=True)
-    print(f"  relL2 vs fp32 oracle {rel_l2(ax, oax):.2e}, vs fp64 oracle {rel_l2(ax, ax64):.2e}")
-    ctx.close()
+    try:
+        for shape in PROJECTOR_SHAPES:
+            print(f"  projector shape {shape}", flush=True)
+            ctx.set_projector_tile(shape)
+            ax, rl, w = ctx.forward_project(0, 0, raylen=rl0)
+            assert np.array_equal(w, ow), f"shape {shape}: sample counts differ at {np.count_nonzero(w != ow)} rays"
+            assert np.array_equal(rl, orl), f"shape {shape}: ray lengths differ"
+            assert rel_l2(ax, oax) <= 1e-5, (shape, rel_l2(ax, oax))
+            print(f"  relL2 vs fp32 oracle {rel_l2(ax, oax):.2e}, vs fp64 oracle {rel_l2(ax, ax64):.2e}")
+    finally:
+        ctx.set_projector_tile(0)
+        ctx.close()
 
 
 def test_fp_constant_volume_closed_form():
@@ -184,15 +193,20 @@ def _setup_pair(vol, det, P, angles_fn, params, F=1, seed=21, flows=True, x0=Non
     return g, ctx, oc
 
 
-def test_sart_sweep_matches_oracle():
-    print("[sart] 32^3 det 48x48 16 views", flush=True)
+@pytest.mark.parametrize("shape", [0, 16])
+def test_sart_sweep_matches_oracle(shape):
+    print(f"[sart] 32^3 det 48x48 16 views projector shape {shape}", flush=True)
     p = stt.params_default()
     p.alpha = 0.3
     ang = list(np.linspace(0, 360, 16, endpoint=False))
     g, ctx, oc = _setup_pair((32, 32, 32), (48, 48), (16,), lambda f: ang, p)
     assert abs(ctx.gmax() - oc.gmax()) <= 1e-6 * abs(oc.gmax())
     order = np.random.default_rng(3).permutation(16).astype(np.int32)
-    ctx.sart_sweep(0, order)
+    try:
+        ctx.set_projector_tile(shape)
+        ctx.sart_sweep(0, order)
+    finally:
+        ctx.set_projector_tile(0)
     oc.sart(0, order)
     got, ref = ctx.get_volume(0), oc.get_volume(0)
     assert rel_l2(got, ref) <= 1e-4, rel_l2(got, ref)
