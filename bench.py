@@ -92,17 +92,17 @@ def copy_calibration(mib):
     log(f"copy calibration: {mib} MiB device-to-device")
 
 
-def load_traffic(config):
+def load_traffic(config, shape):
     """HBM traffic per projector launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated against a copy kernel),
     or None when no summary matches this configuration."""
-    path = os.path.join(ROOT, "profiles", f"traffic_k_fp_level_{config}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_projector_{config}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t
+    return t if t.get("shape") == shape else None
 
 
 def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
@@ -215,7 +215,7 @@ def main():
     iter_bytes = F * P * (12 * V + 12 * D) + F * 124 * V
     ms_per_step = elapsed / args.steps * 1e3
 
-    traffic = load_traffic(args.config) if batched else None
+    traffic = load_traffic(args.config, tile) if batched else None
     out = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -239,7 +239,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_fp_level<SART,{tile}> (forward projection + fused SART correction, all frames' view k)",
+            "kernel": (f"{'k_fp_slab<SART>' if tile == 0 else f'k_fp_level<SART,{tile}>'} "
+                       "(forward projection + fused SART correction, all frames' view k)"),
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
