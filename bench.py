@@ -35,25 +35,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# angles: 0 even over 360, 1 golden angle, 2 van der Corput; P may be a list
+# of per-frame view counts (uneven batches)
 CONFIGS = {
-    "st256": dict(vol=(256, 256, 256), det=(512, 512), F=8, P=45, amp=4.0),
-    "small": dict(vol=(64, 64, 64), det=(128, 128), F=2, P=8, amp=2.0),
+    "st256": dict(vol=(256, 256, 256), det=(512, 512), F=8, P=45, amp=4.0, angles=1),
+    "small": dict(vol=(64, 64, 64), det=(128, 128), F=2, P=8, amp=2.0, angles=1),
+    # 32 frames with seeded view counts U{8..64} (seed 7), 384x384 detector
+    "uneven128": dict(vol=(128, 128, 128), det=(384, 384), F=32,
+                      P=[int(v) for v in np.random.default_rng(7).integers(8, 65, 32)], amp=3.0, angles=2),
+    # full scale: 512^3 x 16 frames, 1024x1024 detector, 60 low-discrepancy views per frame
+    "st512": dict(vol=(512, 512, 512), det=(1024, 1024), F=16, P=60, amp=4.0, angles=2),
 }
 
 
+def views_of(cfg, f):
+    P = cfg["P"]
+    return P[f] if isinstance(P, list) else P
+
+
 def setup(stt, cfg):
-    vol, det, F, P = cfg["vol"], cfg["det"], cfg["F"], cfg["P"]
+    vol, det, F = cfg["vol"], cfg["det"], cfg["F"]
     g = stt.Geometry.make(vol, det, (1.0, 1.0, 1.0), 1.0, 600.0, 1000.0)
     p = stt.params_default()
     p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
     p.bp = stt.STT_BP_VOXEL
     p.prior = stt.STT_PRIOR_STV
     p.interp = stt.STT_INTERP_CUBIC_CLAMPED
-    ctx = stt.Context(g, p, [P] * F)
-    angles = [stt.synth_angles(1, P, offset=P * f) for f in range(F)]
+    Ps = [views_of(cfg, f) for f in range(F)]
+    ctx = stt.Context(g, p, Ps)
+    angles = [stt.synth_angles(cfg["angles"], Ps[f], offset=sum(Ps[:f])) for f in range(F)]
     projections = []
     for f in range(F):
         t0 = time.time()
+        P = Ps[f]
         truth = stt.synth_phantom_frame(vol, f, n_ellipsoids=10, seed=1, deform_seed=2, amp=cfg["amp"])
         ctx.set_views(f, angles[f])
         ctx.set_volume(f, truth)
@@ -111,8 +125,11 @@ def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
-    vol, F, P = cfg["vol"], cfg["F"], cfg["P"]
-    oc = O.OracleCtx(g, p, [P, P])
+    vol, F = cfg["vol"], cfg["F"]
+    Ps = [views_of(cfg, f) for f in range(F)]
+    P = Ps[1]
+    nviews = min(nviews, P)
+    oc = O.OracleCtx(g, p, [Ps[0], P])
     for f in range(2):
         oc.set_views(f, angles[f])
         oc.set_projections(f, projections[f].reshape(-1))
@@ -126,7 +143,7 @@ def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
     rc = O.orc.o_ctx_time_sample(oc.h, 1, nviews, int(faithful), C.byref(t_pd), C.byref(t_v))
     assert rc == 0
     per_view = t_v.value / nviews
-    t_iter = F * (t_pd.value + P * per_view)
+    t_iter = F * t_pd.value + sum(Ps) * per_view
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": 1.0 / t_iter,
@@ -135,7 +152,7 @@ def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
         "kind": "port",
         "variant": "reference-faithful" if faithful else "lean",
         "sample": (f"frame 1 of {F}: PD dual+primal with both warps ({t_pd.value:.2f}s) and {nviews} of {P} "
-                   f"SART views ({per_view:.2f}s/view), scaled x{F} frames x{P} views"),
+                   f"SART views ({per_view:.2f}s/view), scaled to {F} frames and {sum(Ps)} views"),
     }
 
 
@@ -205,16 +222,19 @@ def main():
     ctx.synchronize()
     log(f"kernel timing: {ksteps} eager iterations")
 
-    vol, det, F, P = cfg["vol"], cfg["det"], cfg["F"], cfg["P"]
+    vol, det, F = cfg["vol"], cfg["det"], cfg["F"]
+    Ps = [views_of(cfg, f) for f in range(F)]
     V = int(np.prod(vol))
     D = det[0] * det[1]
-    frames_per_fp = F if batched else 1
+    # frame-views per projector launch: all frames' view k (batched), one (plain)
+    frames_per_fp = sum(Ps) / max(Ps) if batched else 1
     fp_avg_s = fp_ms / max(fp_n, 1) / 1e3
     fp_bytes = frames_per_fp * (4 * V + 8 * D)  # volume read once + b read + correction write
     achieved = fp_bytes / fp_avg_s
-    iter_bytes = F * P * (12 * V + 12 * D) + F * 124 * V
+    iter_bytes = sum(Ps) * (12 * V + 12 * D) + F * 124 * V
     ms_per_step = elapsed / args.steps * 1e3
 
+    views_desc = str(Ps[0]) if len(set(Ps)) == 1 else f"{min(Ps)}-{max(Ps)} (seeded, total {sum(Ps)})"
     traffic = load_traffic(args.config, tile) if batched else None
     out = {
         "metric": METRIC,
@@ -231,10 +251,10 @@ def main():
         "data": "synthetic (seeded deformed ellipsoid phantom, projections = its forward projections)",
         "config": {
             "workload": (f"space-time PD+SART reconstruction {vol[0]}x{vol[1]}x{vol[2]} fp32 x {F} frames, "
-                         f"{P} golden-angle views/frame, {det[0]}x{det[1]} detector, STV prior, voxel BP, "
-                         f"clamped-cubic warp"),
+                         f"{views_desc} {['even', 'golden-angle', 'van der Corput'][cfg['angles']]} views/frame, "
+                         f"{det[0]}x{det[1]} detector, STV prior, voxel BP, clamped-cubic warp"),
             "form": args.form,
-            "volume": list(vol), "frames": F, "views_per_frame": P, "detector": list(det),
+            "volume": list(vol), "frames": F, "views_per_frame": cfg["P"], "detector": list(det),
             "projector_tile": tile,
         },
         "roofline": {
@@ -258,7 +278,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         t1 = time.time()
-        nv = args.cpu_views if args.cpu_views > 0 else cfg["P"]
+        nv = args.cpu_views if args.cpu_views > 0 else views_of(cfg, 1)
         out["cpu_baseline"] = cpu_baseline(g, p, cfg, angles, projections, nv, faithful=True)
         log(f"cpu baseline (faithful) {out['cpu_baseline']['value']:.5f} it/s in {time.time() - t1:.1f}s")
         if args.cpu_lean:

@@ -325,6 +325,8 @@ int valid_geometry(const stt_geometry* g) {
     if (!(g->ds > 0) || !(g->sdd > 0) || !(g->sid > 0)) return fail(STT_ERR_INVALID, "ds, sid, sdd must be > 0");
     const double v = (double)g->vol[0] * g->vol[1] * g->vol[2];
     if ((v + tail_pad(g->vol)) * 4.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "a volume must stay below 4 GiB");
+    // a flow field's three planes are addressed through one 32-bit buffer range
+    if (v * 12.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "a 3-plane flow field must stay below 4 GiB");
     if ((double)g->det[0] * g->det[1] * 4.0 >= 4294967295.0) return fail(STT_ERR_INVALID, "detector too large");
     if ((double)g->vol[0] * g->vol[1] * 4.0 >= 16777216.0 || (double)g->vol[2] * g->vol[1] * 4.0 >= 16777216.0)
         return fail(STT_ERR_INVALID, "a volume slice must stay below 4M voxels (24-bit offsets)");

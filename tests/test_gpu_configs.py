@@ -1,0 +1,146 @@
+"""The benchmark configurations as parity cases on the MI355X.
+
+- **Configuration 1.** Plain single-frame SART+TV, with the oracle run in
+  fp32 and fp64:
+  - 64^3, 10-ellipsoid phantom (seed 1);
+  - 64 even views over 360 degrees, 128x128 detector;
+  - alpha 0.3, lambda 0.01, 20 iterations, identity view order.
+- **Configuration 3.** The differentiable case at its real size:
+  - 128^3 x 10 frames, 32 views/frame on a 256x256 detector;
+  - projector-adjoint and warp-adjoint dot products;
+  - a seeded smooth flow with |u| <= 3.
+- **Configuration 4.** Uneven batched solves: 32 frames with seeded view
+  counts U{8..64} (seed 7), one batched hipGraph ST iteration against the
+  oracle. The volume is reduced to 40^3 (384x384 detector kept) so the CPU
+  oracle finishes in seconds.
+
+  Mixed 128^3/192^3 volumes in one launch are not covered. A context holds
+  one grid, because the PD coupling warps frame f+1 onto frame f.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import spacetimetomography_amd as stt
+from helpers import rel_l2, smooth_flow
+
+pytestmark = pytest.mark.gpu
+
+
+def test_config1_plain_sart_tv_20_iterations():
+    print("[config1] 64^3, 64 even views, 128x128, alpha 0.3, lambda 0.01, 20 iterations", flush=True)
+    dims, det = (64, 64, 64), (128, 128)
+    g = stt.Geometry.make(dims, det, (1, 1, 1), 1.0, 600.0, 1000.0)
+    p = stt.params_default()
+    # the reference's effective solver constants (sigma 0.1125, tau 1) with
+    # k = e = 0 (no temporal terms) and huber 0.1 (main.cpp:73)
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.1125, 1.0, 0.01, 0.0, 0.0, 0.1, 0.3
+    P = 64
+    ang = list(stt.synth_angles(0, P))
+    truth = stt.synth_phantom(dims, 10, seed=1)
+    ctx = stt.Context(g, p, [P])
+    ctx.set_views(0, ang)
+    ctx.set_volume(0, truth)
+    b = np.stack([ctx.forward_project(0, k)[0] for k in range(P)])
+    zero = np.zeros_like(truth)
+    ctx.set_volume(0, zero)
+    ctx.set_volume(1, zero)
+    ctx.set_projections(0, b)
+    ocs = []
+    for f64 in (False, True):
+        oc = O.OracleCtx(g, p, [P], f64mode=f64)
+        oc.set_views(0, ang)
+        oc.set_volume(0, zero)
+        oc.set_volume(1, zero)
+        oc.set_projections(0, b.reshape(-1))
+        ocs.append(oc)
+    ctx.pd_init()
+    ctx.iterate(20, batched=False)
+    got = ctx.get_volume(0)
+    for oc in ocs:
+        oc.pd_init()
+        oc.iterate(20)
+    ref32, ref64 = ocs[0].get_volume(0), ocs[1].get_volume_f64(0)
+    e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
+    print(f"  relL2 vs fp32 oracle {e32:.2e}, vs fp64 oracle {e64:.2e}, fp32 oracle vs fp64 {rel_l2(ref32, ref64):.2e}, "
+          f"reconstruction error vs phantom {rel_l2(got, truth):.3f}", flush=True)
+    assert e32 <= 1e-4, e32
+    assert rel_l2(got, truth) < rel_l2(zero, truth)  # it reconstructs
+    ctx.close()
+
+
+def test_config3_adjoint_dot_products_at_128():
+    print("[config3] 128^3 x 10 frames, 32 views, 256x256: FP adjoint and warp adjoint dot tests", flush=True)
+    dims, det = (128, 128, 128), (256, 256)
+    V = int(np.prod(dims))
+    g = stt.Geometry.make(dims, det, (1, 1, 1), 1.0, 600.0, 1000.0)
+    F, P = 10, 32
+    ctx = stt.Context(g, stt.params_default(), [P] * F)
+    rng_x, rng_y = np.random.default_rng(5), np.random.default_rng(6)
+    worst = 0.0
+    for f in (0, 9):  # first and last frame of the stack, 4 views each
+        ctx.set_views(f, list(stt.synth_angles(1, P, offset=P * f)))
+        x = rng_x.standard_normal(V).astype(np.float32)
+        ctx.set_volume(f, x)
+        for k in range(0, P, 8):
+            ax = ctx.forward_project(f, k)[0]
+            y = rng_y.standard_normal(det[0] * det[1]).astype(np.float32)
+            aty = ctx.fp_vjp(f, k, y)
+            lhs = float(np.dot(ax.astype(np.float64), y))
+            rhs = float(np.dot(x.astype(np.float64), aty))
+            rel = abs(lhs - rhs) / (np.linalg.norm(ax) * np.linalg.norm(y))
+            worst = max(worst, rel)
+    print(f"  projector: worst |<Ax,y> - <x,A^T y>| / (|Ax||y|) = {worst:.2e}", flush=True)
+    assert worst <= 1e-6
+    uvw = smooth_flow(dims, 3.0, seed=4)
+    src = rng_x.standard_normal(V).astype(np.float32)
+    gout = rng_y.standard_normal(V).astype(np.float32)
+    for dt in (-1.0, 1.0):
+        wsrc = ctx.warp(uvw, src, dt, stt.STT_INTERP_TRILINEAR)
+        gsrc, _ = ctx.warp_vjp(uvw, src, dt, gout)
+        lhs = float(np.dot(wsrc.astype(np.float64), gout))
+        rhs = float(np.dot(src.astype(np.float64), gsrc))
+        rel = abs(lhs - rhs) / (np.linalg.norm(wsrc) * np.linalg.norm(gout))
+        print(f"  warp dt={dt}: |<Wx,g> - <x,W^T g>| / (|Wx||g|) = {rel:.2e}", flush=True)
+        assert rel <= 1e-6
+    ctx.close()
+
+
+def test_config4_uneven_batched_iteration():
+    rng = np.random.default_rng(7)
+    F = 32
+    P = [int(v) for v in rng.integers(8, 65, F)]
+    dims, det = (40, 40, 40), (384, 384)
+    print(f"[config4] F={F} views U{{8..64}} (total {sum(P)}), {dims}, {det}, batched + hipGraph", flush=True)
+    g = stt.Geometry.make(dims, det, (1, 1, 1), 0.25, 600.0, 1000.0)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    ctx = stt.Context(g, p, P)
+    oc = O.OracleCtx(g, p, P)
+    zero = np.zeros(int(np.prod(dims)), np.float32)
+    for f in range(F):
+        ang = list(stt.synth_angles(2, P[f], offset=sum(P[:f])))
+        truth = stt.synth_phantom_frame(dims, f, n_ellipsoids=6, seed=8, deform_seed=2, amp=1.5)
+        ctx.set_views(f, ang)
+        oc.set_views(f, ang)
+        ctx.set_volume(f, truth)
+        b = np.stack([ctx.forward_project(f, k)[0] for k in range(P[f])])
+        ctx.set_projections(f, b)
+        oc.set_projections(f, b.reshape(-1))
+        if f < F - 1:
+            u = stt.synth_flow(dims, f, deform_seed=2, amp=1.5)
+            ctx.set_flow(f, u)
+            oc.set_flow(f, u)
+    for f in range(F + 1):
+        ctx.set_volume(f, zero)
+        oc.set_volume(f, zero)
+    ctx.pd_init()
+    oc.pd_init()
+    ctx.autotune()
+    ctx.iterate(1, batched=True, use_graph=True)
+    oc.per_frame_raylen(True)
+    oc.iterate(1)
+    worst = max(rel_l2(ctx.get_volume(f), oc.get_volume(f)) for f in range(F))
+    print(f"  worst frame relL2 vs oracle {worst:.2e}", flush=True)
+    assert worst <= 1e-4
+    ctx.close()
