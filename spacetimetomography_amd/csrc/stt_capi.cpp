@@ -81,6 +81,16 @@ struct stt_ctx {
     WorkItem* items_plain = nullptr;
     WorkItem* items_batch = nullptr;
     std::vector<int> batch_off, batch_cnt;
+    // the batched sweep split into frame groups, one per stream: group h
+    // holds frames [split_f[h], split_f[h+1]), items in split_items, per k
+    static constexpr int kGroups = 4;
+    WorkItem* split_items = nullptr;
+    int ngroups = 1;
+    int split_f[kGroups + 1] = {};
+    std::vector<int> split_off[kGroups], split_cnt[kGroups];
+    int split_maxP[kGroups] = {};
+    hipStream_t streams[kGroups] = {};  // streams[0] is the context stream
+    hipEvent_t ev_fork = nullptr, ev_join[kGroups] = {};
     WorkItem* items_scratch = nullptr;  // ad-hoc single launches
     int scratch_cap = 0;
     bool tables_dirty = true;
@@ -92,6 +102,11 @@ struct stt_ctx {
     ~stt_ctx() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (void* a : allocs) (void)hipFree(a);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (int h = 1; h < kGroups; ++h) {
+            if (ev_join[h]) (void)hipEventDestroy(ev_join[h]);
+            if (streams[h]) (void)hipStreamDestroy(streams[h]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -171,6 +186,29 @@ struct stt_ctx {
             batch_cnt[k] = (int)batch.size() - batch_off[k];
         }
         HIPCHK(hipMemcpyAsync(items_batch, batch.data(), batch.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+        // frame groups with about equal view counts (contiguous frames)
+        ngroups = std::min(kGroups, F);
+        split_f[0] = 0;
+        for (int h = 1, f = 0, acc = 0; h < ngroups; ++h) {
+            while (f < F - (ngroups - h) && (long)acc * ngroups < (long)h * total_views) acc += P[f++];
+            split_f[h] = std::max(f, split_f[h - 1] + 1);
+            f = split_f[h];
+        }
+        split_f[ngroups] = F;
+        std::vector<WorkItem> split;
+        for (int h = 0; h < ngroups; ++h) {
+            split_maxP[h] = 0;
+            for (int f = split_f[h]; f < split_f[h + 1]; ++f) split_maxP[h] = std::max(split_maxP[h], P[f]);
+            split_off[h].assign(split_maxP[h], 0);
+            split_cnt[h].assign(split_maxP[h], 0);
+            for (int k = 0; k < split_maxP[h]; ++k) {
+                split_off[h][k] = (int)split.size();
+                for (int f = split_f[h]; f < split_f[h + 1]; ++f)
+                    if (k < P[f]) split.push_back(make_item(f, k, false));
+                split_cnt[h][k] = (int)split.size() - split_off[h][k];
+            }
+        }
+        HIPCHK(hipMemcpyAsync(split_items, split.data(), split.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
         std::vector<FrameBufs> fbs(F + 1);
         for (int f = 0; f <= F; ++f) {
             fbs[f].x = x[f];
@@ -253,7 +291,35 @@ struct stt_ctx {
         auto mark = [&](int k) {
             if (tm) tm->mark(k);
         };
-        if (batched) {
+        if (batched && !tm && ngroups >= 2) {
+            // The frames' sweeps are independent: the frame groups run on
+            // their own streams (default priority), so one group's
+            // backprojection overlaps another group's projector.  Joined
+            // before returning.
+            HIPCHK(launch_pd_dual(fb_dev, 0, F, vg, pc, stream));
+            HIPCHK(launch_pd_primal(fb_dev, 0, F, vg, pc, stream));
+            HIPCHK(hipEventRecord(ev_fork, stream));
+            for (int h = 1; h < ngroups; ++h) HIPCHK(hipStreamWaitEvent(streams[h], ev_fork, 0));
+            for (int h = 0; h < ngroups; ++h) {
+                hipStream_t sh = streams[h];
+                for (int k = 0; k < split_maxP[h]; ++k) {
+                    const WorkItem* it = split_items + split_off[h][k];
+                    const int n = split_cnt[h][k];
+                    HIPCHK(launch_transpose_xz(it, n, vg, sh));
+                    HIPCHK(launch_fp(FP_MODE_SART, it, n, vg, pc.factor, sh));
+                    if (ray_bp) {
+                        HIPCHK(launch_fp(FP_MODE_SCATTER, it, n, vg, 1.0f, sh));
+                        HIPCHK(launch_bp_ray_apply(it, n, vg, pc, sh));
+                    } else {
+                        HIPCHK(launch_bp_voxel(it, n, vg, pc, sh));
+                    }
+                }
+            }
+            for (int h = 1; h < ngroups; ++h) {
+                HIPCHK(hipEventRecord(ev_join[h], streams[h]));
+                HIPCHK(hipStreamWaitEvent(stream, ev_join[h], 0));
+            }
+        } else if (batched) {
             HIPCHK(launch_pd_dual(fb_dev, 0, F, vg, pc, stream));
             mark(3);
             HIPCHK(launch_pd_primal(fb_dev, 0, F, vg, pc, stream));
@@ -393,8 +459,14 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
         delete c;
         return code;
     };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(STT_ERR_HIP, "hipStreamCreate failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(STT_ERR_HIP, "stream or event creation failed"));
+    c->streams[0] = c->stream;
+    for (int h = 1; h < stt_ctx::kGroups; ++h)
+        if (hipStreamCreateWithFlags(&c->streams[h], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join[h], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(STT_ERR_HIP, "stream or event creation failed"));
     c->V = (size_t)g->vol[0] * g->vol[1] * g->vol[2];
     c->Vpad = c->V + tail_pad(g->vol);
     const int volT[3] = {g->vol[2], g->vol[1], g->vol[0]};
@@ -455,6 +527,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     if ((rc = c->alloc_t(&c->fb_dev, (size_t)F + 1))) return bail(rc);
     if ((rc = c->alloc_t(&c->items_plain, (size_t)c->total_views))) return bail(rc);
     if ((rc = c->alloc_t(&c->items_batch, (size_t)c->total_views))) return bail(rc);
+    if ((rc = c->alloc_t(&c->split_items, (size_t)c->total_views))) return bail(rc);
     c->scratch_cap = std::max(std::max(64, c->maxP + F), 4 * F);  // 4*F: the tuner's batched launches
     if ((rc = c->alloc_t(&c->items_scratch, (size_t)c->scratch_cap))) return bail(rc);
     c->views_host.resize(c->total_views);
