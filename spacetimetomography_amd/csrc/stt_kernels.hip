@@ -962,8 +962,8 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 // centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
 // clamp [0, G_Max].  Atomic-free, one thread per voxel column segment.
 //
-// A 256-thread block owns a 64 x 4 x 8 voxel tile (lane = x, wave = y, each
-// thread 8 consecutive z).  The tile is a box, so the detector positions of
+// A 256-thread block owns a 64 x 4 x 16 voxel tile (lane = x, wave = y, each
+// thread 16 consecutive z).  The tile is a box, so the detector positions of
 // its voxels lie inside the bounding box of its eight corners' projections;
 // the block stages that box of the correction image in LDS with row loads and
 // reads every voxel's 4x4 window from there: 16 taps per voxel from LDS
@@ -978,8 +978,8 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 // window read from global memory when it is not inside the staged box.
 constexpr int kBpTileX = 64;
 constexpr int kBpTileY = 4;
-constexpr int kBpVoxZ = 8;  // voxels per thread along z
-constexpr int kBpLdsFloats = 4096;  // 16 KB staged detector footprint per block
+constexpr int kBpVoxZ = 16;  // voxels per thread along z (8 and 32 measured slower)
+constexpr int kBpLdsFloats = 5632;  // 22 KB staged detector footprint per block
 constexpr int kBpMaxCols = 192;     // three 64-lane column chunks
 constexpr int kBpMaxRows = 64;      // 16 rows per wave
 
