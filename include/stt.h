@@ -233,6 +233,20 @@ int  stt_residual_norm(stt_ctx* ctx, double* r2);
  * whole volume with v[F] = 0).  *L = sqrt(lambda_max(K^T K)) estimate. */
 int  stt_power_iteration(stt_ctx* ctx, int iters, double* L);
 
+/* ---- acquisition view tables (host only, no device needed) -------------- */
+/* The reference's rounds of views (InitViews, ST_Tomography.cpp:887-934 with
+ * RotateVolume :865-884): round j starts at start_deg[j] and each further
+ * view rotates the previous one by 360/views_per_round degrees about +y,
+ * composed in fp32 as the reference does.  Writes nrounds*views_per_round
+ * quaternions (x,y,z,w), ready for stt_set_view_quaternions. */
+int  stt_view_rounds(const float* start_deg, int nrounds, int views_per_round,
+                     float* q_xyzw);
+/* Scanning angles of Low_discrepency_squence(Np, Ns, p, Nz)
+ * (low discrepancy sequence/Low_discrepency_squence.m:15-44): van der
+ * Corput radical inverse in base p over ns circle segments, plus a 0-degree
+ * projection before every nz-th entry.  Writes np angles in degrees. */
+int  stt_lds_sequence(int np, int ns, int p, int nz, float* out_deg);
+
 /* ---- host-side synthetic inputs (no device needed) ----------------------- */
 /* Additive ellipsoid phantom on an nx*ny*nz grid (seeded). */
 int  stt_synth_phantom(int nx, int ny, int nz, int n_ellipsoids, uint64_t seed,

@@ -111,6 +111,8 @@ _SIGS = {
                                           C.c_float, C.c_int, _F]),
     "stt_synth_flow": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_int, _F]),
     "stt_synth_angles": (C.c_int, [C.c_int, C.c_int, C.c_int, _F]),
+    "stt_view_rounds": (C.c_int, [_F, C.c_int, C.c_int, _F]),
+    "stt_lds_sequence": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _F]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -181,6 +183,22 @@ def synth_flow(dims, t, deform_seed=2, amp=4.0):
 def synth_angles(kind, n, offset=0):
     out = np.empty(n, np.float32)
     check(lib.stt_synth_angles(kind, n, offset, fptr(out)))
+    return out
+
+
+def view_rounds(start_deg, views_per_round=10):
+    """Quaternions (n, 4) of the reference's rounds of views: each round from
+    its start angle, then views_per_round-1 rotations by 360/views_per_round."""
+    st = np.ascontiguousarray(start_deg, np.float32).reshape(-1)
+    out = np.zeros((len(st) * views_per_round, 4), np.float32)
+    check(lib.stt_view_rounds(fptr(st), len(st), views_per_round, fptr(out)))
+    return out
+
+
+def lds_sequence(np_, ns=1, p=2, nz=10**9):
+    """Low_discrepency_squence(Np, Ns, p, Nz) angles in degrees."""
+    out = np.zeros(np_, np.float32)
+    check(lib.stt_lds_sequence(np_, ns, p, nz, fptr(out)))
     return out
 
 

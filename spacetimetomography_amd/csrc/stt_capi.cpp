@@ -1032,6 +1032,44 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     return STT_OK;
 }
 
+int stt_view_rounds(const float* start_deg, int nrounds, int views_per_round, float* q_xyzw) {
+    if (!start_deg || !q_xyzw || nrounds < 1 || views_per_round < 1)
+        return fail(STT_ERR_INVALID, "need start angles, rounds >= 1, views per round >= 1");
+    // InitViews (ST_Tomography.cpp:887-934): each round starts from the
+    // identity rotated by its start angle; every further view of the round
+    // rotates the previous one by 360/views_per_round degrees (RotateVolume,
+    // degreeEachProj = 360.0/projsEachRound, :140-141, :865-884).
+    const double step = 360.0 / views_per_round;
+    for (int j = 0; j < nrounds; ++j) {
+        float* q = q_xyzw + 4 * (size_t)j * views_per_round;
+        quat_about_y(start_deg[j], q);
+        for (int k = 1; k < views_per_round; ++k) quat_rotate_y(q + 4 * (k - 1), step, q + 4 * k);
+    }
+    return STT_OK;
+}
+
+int stt_lds_sequence(int np, int ns, int p, int nz, float* out_deg) {
+    if (!out_deg || np < 0 || ns < 1 || p < 2 || nz < 1) return fail(STT_ERR_INVALID, "need np >= 0, ns >= 1, p >= 2, nz >= 1");
+    // Low_discrepency_squence.m:15-44: the radical inverse of i = 0, 1, ...
+    // in base p scaled to one of ns circle segments, replicated over the ns
+    // segments, with an extra 0-degree projection before every nz-th entry.
+    int len = 0;
+    for (long long i = 0; len < np; ++i) {
+        if (len > 0 && len % nz == 0) {
+            out_deg[len++] = 0.0f;
+            if (len >= np) break;
+        }
+        double r = 0.0, q = 1.0 / p;
+        for (long long b = i; b != 0; b /= p) {
+            r += (double)(b % p) * q;
+            q /= p;
+        }
+        r = 360.0 * r / ns;
+        for (int k = 0; len < np && k < ns; ++k) out_deg[len++] = (float)(r + k * 360.0 / ns);
+    }
+    return STT_OK;
+}
+
 int stt_synchronize(stt_ctx* ctx) {
     CTX_CHECK(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));

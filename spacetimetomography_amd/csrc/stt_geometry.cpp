@@ -134,6 +134,21 @@ void quat_about_y(float theta_deg, float q[4]) {
     q[2] = w * 0.0f + ax[2] * 1.0f + ax[0] * 0.0f - ax[1] * 0.0f;
 }
 
+void quat_rotate_y(const float q_old[4], double deg, float q[4]) {
+    // RotateVolume (ST_Tomography.cpp:865-884): Quaternion(0,1,0, angle) *
+    // m_OldRotation with angle = deg*3.1415926/180 taken to float, and the
+    // Hamilton product in the order of Math/Quaternion.h:76-86.
+    const float angle = (float)(deg * 3.1415926 / 180);
+    const float half = angle / 2;
+    const float pw = std::cos(half), ps = std::sin(half);
+    const float px = 0.0f * ps, py = 1.0f * ps, pz = 0.0f * ps;
+    const float qx = q_old[0], qy = q_old[1], qz = q_old[2], qw = q_old[3];
+    q[3] = pw * qw - px * qx - py * qy - pz * qz;
+    q[0] = pw * qx + px * qw + py * qz - pz * qy;
+    q[1] = pw * qy + py * qw + pz * qx - px * qz;
+    q[2] = pw * qz + pz * qw + px * qy - py * qx;
+}
+
 ViewRec build_view(const stt_geometry& g, const float q[4]) {
     // Camera of the ST_Tomography ctor and Scene::_calculateCamera
     // (ST_Tomography.cpp:229-246; Scene.cpp:77,131,133).

@@ -8,6 +8,8 @@ namespace stt {
 // Quaternion (x, y, z, w) of a rotation by theta degrees about +y, as the
 // reference's InitViews builds it.
 void quat_about_y(float theta_deg, float q[4]);
+// Rotation by deg degrees about +y composed with q_old (RotateVolume).
+void quat_rotate_y(const float q_old[4], double deg, float q[4]);
 // Per-view geometry record (fp32, reference arithmetic order).
 ViewRec build_view(const stt_geometry& g, const float q[4]);
 // Sample distance ||spacing||.
