@@ -633,21 +633,21 @@ struct SlabBox {
     bool staged;
 };
 
-// Block-wide min/max of five values (wave shuffles, then one LDS round).
-template <int NW>
-__device__ __forceinline__ void block_minmax5(float v[5], const bool is_max[5], float* red) {
+// Block-wide min/max of NV values (wave shuffles, then one LDS round).
+template <int NW, int NV>
+__device__ __forceinline__ void block_minmax(float v[NV], const bool is_max[NV], float* red) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < NV; ++k) {
         v[k] = is_max[k] ? wave_max_f32(v[k]) : wave_min_f32(v[k]);
-        if (lane == 0) red[wv * 5 + k] = v[k];
+        if (lane == 0) red[wv * NV + k] = v[k];
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < NV; ++k) {
         float a = red[k];
 #pragma unroll
-        for (int w = 1; w < NW; ++w) a = is_max[k] ? fmaxf(a, red[w * 5 + k]) : fminf(a, red[w * 5 + k]);
+        for (int w = 1; w < NW; ++w) a = is_max[k] ? fmaxf(a, red[w * NV + k]) : fminf(a, red[w * NV + k]);
         v[k] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a)));
     }
 }
@@ -655,15 +655,18 @@ __device__ __forceinline__ void block_minmax5(float v[5], const bool is_max[5], 
 // Box of the slab whose sample base planes start at c0 (uniform arithmetic).
 template <int D, bool DOWN>
 __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const float o[3], float sAmin, float sAmax,
-                                            float sBmin, float sBmax) {
+                                            float sBmin, float sBmax, float dfirst) {
     constexpr int A = D == 0 ? 1 : 0;
     constexpr int B = D == 2 ? 1 : 2;
     const int dims[3] = {g.nx, g.ny, g.nz};
     SlabBox bx;
     int K = kSlabK;
     // positions of the slab's samples along D: [c0, c0 + K) going up,
-    // [c0 - K + 1, c0 + 1) going down; transverse extremes at the two ends
-    const float e0 = (float)(DOWN ? c0 + 1 : c0) - o[D];
+    // [c0 - K + 1, c0 + 1) going down, transverse extremes at the two ends.
+    // Samples before a ray's entry point lie outside the volume along D (the
+    // clamp maps them to the first plane); dfirst, the block's extreme first
+    // sample along D, extends the near end over them.
+    const float e0 = (DOWN ? fmaxf((float)(c0 + 1), dfirst) : fminf((float)c0, dfirst)) - o[D];
     const float e1 = (float)(DOWN ? c0 - K + 1 : c0 + K) - o[D];
     const float a0 = fminf(fminf(e0 * sAmin, e0 * sAmax), fminf(e1 * sAmin, e1 * sAmax)) + o[A];
     const float a1 = fmaxf(fmaxf(e0 * sAmin, e0 * sAmax), fmaxf(e1 * sAmin, e1 * sAmax)) + o[A];
@@ -776,11 +779,20 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         fetch_taps(vv, e0, e1, e2, T);
         density = lerp_taps(T) * g.sd;
         n = 1;
+    } else if (have) {
+        // The entry sample coincides with the pending one and counts twice
+        // (weight 2); its first copy is taken here, so every sample of the
+        // marching loops has weight 1.
+        Taps T;
+        fetch_taps(vv, G.pend[0], G.pend[1], G.pend[2], T);
+        density = lerp_taps(T) * g.sd;
+        n = 1;
+        G.w = 1.0f;
     }
     // Slopes of the rays that march with the slabs, and the first base plane.
     const bool fwd = DOWN ? r.d[D] < 0.0f : r.d[D] > 0.0f;
-    float q[5];
-    const bool qmax[5] = {false, true, false, true, DOWN};
+    float q[6];
+    const bool qmax[6] = {false, true, false, true, DOWN, DOWN};
     {
         const float sa = r.d[A] / r.d[D], sb = r.d[B] / r.d[D];
         const bool use = have && fwd;
@@ -790,8 +802,9 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         q[3] = use ? sb : -INFINITY;
         const float pD = floorf(__builtin_amdgcn_fmed3f(G.pend[D], 0.0f, bnd[D]));
         q[4] = have ? pD : (DOWN ? -INFINITY : INFINITY);
+        q[5] = use ? G.pend[D] : (DOWN ? -INFINITY : INFINITY);  // unclamped
     }
-    block_minmax5<NT / 64>(q, qmax, red);
+    block_minmax<NT / 64, 6>(q, qmax, red);
     float o[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r.o[c])));
@@ -800,12 +813,68 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     int c0 = (int)fminf(fmaxf(q[4], -1.0f), (float)g.nx + g.ny + g.nz);
     const int dimD = D == 0 ? g.nx : (D == 1 ? g.ny : g.nz);
     const float sdw = g.sd;
+    // Every sample of a ray that marches with the slabs lies inside the box
+    // of its slab.  The box spans the block's slope range from the common eye
+    // over the slab's extent along D (the first slab's extended back to the
+    // samples before the entry points), plus one cell on each side; a sample
+    // deviates from its exact line by the rounding of at most max_samples
+    // additions (about 1e-2 cells), so its cell never leaves the box.  Waves
+    // whose sampling lanes all march forward take the loop without a
+    // per-sample box test.
+    const bool wave_in_box = __builtin_amdgcn_ballot_w64(have && !fwd) == 0;
+    const float maxn = (float)g.max_samples;
+    float nf = (float)n;
 #pragma unroll 1
     for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
-        SlabBox bx = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3]);
+        // only the first slab holds samples before an entry point
+        const float dfirst = guard == 0 ? q[5] : (DOWN ? -INFINITY : INFINITY);
+        SlabBox bx = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3], dfirst);
         if (!slopes_ok) bx.staged = false;
         if (bx.staged) slab_stage<NT>(vv, g, bx, box);
         __syncthreads();
+        if (bx.staged && wave_in_box) {
+            // Samples of the slab are those whose clamped base plane p = (int)cD
+            // satisfies p < cend (up) or p > cend (down); cD >= 0, so the test
+            // runs on the float.  A lane leaves the loop at its first sample
+            // past the slab, or at max_samples, or when its ray ends (nlim = -1).
+            const float lim = DOWN ? (float)(bx.cend + 1) : (float)bx.cend;
+            const char* bb = (const char*)(box - bx.base);
+            const unsigned px4 = 4u * (unsigned)bx.px, pxy4 = 4u * (unsigned)bx.pxy;
+            float nlim = have ? maxn : -1.0f;
+#pragma unroll 1
+            for (;;) {
+                const float cx = __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx);
+                const float cy = __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by);
+                const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
+                const float cc[3] = {cx, cy, cz};
+                if (!((DOWN ? cc[D] >= lim : cc[D] < lim) && nf < nlim)) break;
+                const unsigned off = ((unsigned)(int)cx << 2) + __umul24((unsigned)(int)cy, px4) +
+                                     __umul24((unsigned)(int)cz, pxy4);
+                const float* b0 = (const float*)(bb + off);
+                Taps T;
+                T.fx = __builtin_amdgcn_fractf(cx);
+                T.fy = __builtin_amdgcn_fractf(cy);
+                T.fz = __builtin_amdgcn_fractf(cz);
+                T.r00 = make_float2(b0[0], b0[1]);
+                T.r10 = make_float2(b0[bx.px], b0[bx.px + 1]);
+                T.r01 = make_float2(b0[bx.pxy], b0[bx.pxy + 1]);
+                T.r11 = make_float2(b0[bx.pxy + bx.px], b0[bx.pxy + bx.px + 1]);
+                nf += 1.0f;
+                G.t += sdw;
+                G.pend[0] += G.step[0];
+                G.pend[1] += G.step[1];
+                G.pend[2] += G.step[2];
+                if (__builtin_expect(!(G.t < G.tb), 0)) {
+                    if (!raygen_next_segment(G, sdw)) nlim = -1.0f;
+                }
+                density = fmaf(lerp_taps(T), sdw, density);
+            }
+            have = nf < nlim;
+            c0 = bx.cend;
+            if (!__syncthreads_or(have)) break;
+            continue;
+        }
+        n = (int)nf;
         // One wave-uniform loop per slab: each iteration every lane whose
         // pending sample lies in the slab (or behind it) takes it.
 #pragma unroll 1
@@ -853,10 +922,11 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
                 density = fmaf(lerp_taps(T), w * sdw, density);
             }
         }
+        nf = (float)n;
         c0 = bx.cend;
         if (!__syncthreads_or(have)) break;
     }
-    nsamp = n;
+    nsamp = (int)nf;
     return density;
 }
 
@@ -864,7 +934,7 @@ template <int MODE, int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__ items, VolGeom g, float factor) {
     constexpr int NT = TW * TH;
     __shared__ float box[kSlabCap];
-    __shared__ float red[5 * NT / 64];
+    __shared__ float red[6 * NT / 64];
     int i, j;
     const bool valid = tile_pixel<TW, TH>(g, i, j);
     const WorkItem& it = items[blockIdx.y];

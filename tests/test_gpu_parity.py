@@ -61,6 +61,41 @@ def test_fp_matches_oracle(case):
         ctx.close()
 
 
+def test_fp_shapes_agree_at_full_size():
+    """256^3 phantom, 512x512 detector (the iteration benchmark's geometry):
+    the slab projector reads its volume from LDS boxes, the level projector
+    straight from HBM.  Views around the 45-degree switch of the dominant axis
+    included.  Sample counts must be identical and values agree to fp32
+    rounding; two views are also checked against the oracle."""
+    print("[fp-full] 256^3, 512x512, slab vs level-16 vs oracle", flush=True)
+    dims, det = (256, 256, 256), (512, 512)
+    angles = [0.0, 17.3, 44.99, 45.0, 45.01, 90.0, 134.9, 180.0, 226.1, 271.7, 315.0, 359.9]
+    g = stt.Geometry.make(dims, det, (1, 1, 1), 1.0, 600.0, 1000.0)
+    ctx = stt.Context(g, stt.params_default(), [len(angles)])
+    x = stt.synth_phantom(dims, 10, seed=3)
+    ctx.set_volume(0, x)
+    ctx.set_views(0, angles)
+    try:
+        for v, a in enumerate(angles):
+            ctx.set_projector_tile(0)
+            ax0, rl0, w0 = ctx.forward_project(0, v)
+            ctx.set_projector_tile(16)
+            ax1, rl1, w1 = ctx.forward_project(0, v)
+            assert np.array_equal(w0, w1), (a, np.count_nonzero(w0 != w1))
+            assert np.array_equal(rl0, rl1), a
+            e = rel_l2(ax0, ax1)
+            assert e <= 1e-6, (a, e)
+            if v in (1, 4):
+                oax, _, ow = O.fp(g, O.quat_y(a), x)
+                assert np.array_equal(w0, ow), a
+                assert rel_l2(ax0, oax) <= 1e-5, (a, rel_l2(ax0, oax))
+                print(f"  angle {a}: slab vs oracle {rel_l2(ax0, oax):.2e}", flush=True)
+            print(f"  angle {a}: slab vs level {e:.2e}", flush=True)
+    finally:
+        ctx.set_projector_tile(0)
+        ctx.close()
+
+
 def test_fp_constant_volume_closed_form():
     """Constant volume c: every hit ray integrates to c * sd * (number of samples)."""
     print("[fp-const] 32^3, 48x48", flush=True)
