@@ -482,6 +482,13 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     c->vg.by = (float)(g->vol[1] - 1);
     c->vg.bz = (float)(g->vol[2] - 1);
     c->vg.max_samples = 2 * (g->vol[0] + g->vol[1] + g->vol[2]) + 16;
+    {
+        // accumulated rounding of pend += step over max_samples steps, with
+        // |pend| <= 2*max(dims) (half an ulp per addition), plus 1/32 cell
+        const float cmax = 2.0f * (float)std::max(g->vol[0], std::max(g->vol[1], g->vol[2]));
+        const float ulp = std::nextafter(cmax, INFINITY) - cmax;
+        c->vg.box_eps = 1.0f / 32.0f + 0.5f * ulp * (float)c->vg.max_samples;
+    }
     c->vg.vol_bytes = (unsigned)(c->Vpad * sizeof(float));
     c->vg.volT_bytes = (unsigned)(c->VpadT * sizeof(float));
     c->vg.det_bytes = (unsigned)((size_t)g->det[0] * g->det[1] * sizeof(float));

@@ -39,6 +39,8 @@ struct VolGeom {
     float sd;            // sample distance ||spacing|| (ST_Tomography.cpp:300)
     float bx, by, bz;    // CroppingBounds upper faces dims-1 (TypicalRayProducer.core.h:241-244)
     int max_samples;     // hard cap of the sample loop (never reached by a valid ray)
+    float box_eps;       // bound (cells) on how far a sample drifts off its exact line: max_samples
+                         // rounded additions at coordinates of magnitude <= 2*max(dims), plus slack
     unsigned vol_bytes;  // bytes of one padded volume allocation (buffer range)
     unsigned volT_bytes; // bytes of one padded transposed (z-fastest) volume
     unsigned det_bytes;  // bytes of one detector image

@@ -622,7 +622,9 @@ __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ i
 // LDS) reads them from global memory.  No layout copy of the volume is needed
 // for any view direction.
 constexpr int kSlabCap = 6144;  // floats of staged box per block (24 KB; 20 and 28 KB measured slower)
-constexpr int kSlabK = 23;      // sample planes per slab (box depth <= 24)
+constexpr int kSlabK = 23;      // most sample planes per slab (box depth <= 24)
+constexpr int kSlabKMin = 8;    // fewest sample planes per slab (before the LDS cap)
+constexpr float kSlabShear = 0.5f;  // slab depth: the box may widen by this fraction of its near-plane width
 constexpr int kStagePasses = 4; // staging covers x-y windows up to 4*256 elements
 
 struct SlabBox {
@@ -660,23 +662,33 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     constexpr int B = D == 2 ? 1 : 2;
     const int dims[3] = {g.nx, g.ny, g.nz};
     SlabBox bx;
-    int K = kSlabK;
     // positions of the slab's samples along D: [c0, c0 + K) going up,
     // [c0 - K + 1, c0 + 1) going down, transverse extremes at the two ends.
     // Samples before a ray's entry point lie outside the volume along D (the
     // clamp maps them to the first plane); dfirst, the block's extreme first
     // sample along D, extends the near end over them.
     const float e0 = (DOWN ? fmaxf((float)(c0 + 1), dfirst) : fminf((float)c0, dfirst)) - o[D];
+    // Depth: an oblique bundle crosses an axis-aligned box diagonally, so the
+    // box's transverse extent grows by the largest |slope| per plane; the slab
+    // ends where that growth reaches kSlabShear of the near-plane width.
+    int K;
+    {
+        const float wA = fabsf(e0) * (sAmax - sAmin) + 2.0f, gA = fmaxf(fabsf(sAmin), fabsf(sAmax));
+        const float wB = fabsf(e0) * (sBmax - sBmin) + 2.0f, gB = fmaxf(fabsf(sBmin), fabsf(sBmax));
+        const float kf = kSlabShear * fminf(wA * __builtin_amdgcn_rcpf(gA), wB * __builtin_amdgcn_rcpf(gB));
+        K = (int)fminf(fmaxf(kf, (float)kSlabKMin), (float)kSlabK);
+    }
     const float e1 = (float)(DOWN ? c0 - K + 1 : c0 + K) - o[D];
     const float a0 = fminf(fminf(e0 * sAmin, e0 * sAmax), fminf(e1 * sAmin, e1 * sAmax)) + o[A];
     const float a1 = fmaxf(fmaxf(e0 * sAmin, e0 * sAmax), fmaxf(e1 * sAmin, e1 * sAmax)) + o[A];
     const float b0 = fminf(fminf(e0 * sBmin, e0 * sBmax), fminf(e1 * sBmin, e1 * sBmax)) + o[B];
     const float b1 = fmaxf(fmaxf(e0 * sBmin, e0 * sBmax), fmaxf(e1 * sBmin, e1 * sBmax)) + o[B];
-    const float lim = 1.0e6f;
-    const int loA = max((int)floorf(fminf(fmaxf(a0, -lim), lim)) - 1, 0);
-    const int loB = max((int)floorf(fminf(fmaxf(b0, -lim), lim)) - 1, 0);
-    const int hiA = min((int)floorf(fminf(fmaxf(a1, -lim), lim)) + 2, dims[A]);
-    const int hiB = min((int)floorf(fminf(fmaxf(b1, -lim), lim)) + 2, dims[B]);
+    // cells of the samples' exact positions widened by box_eps, plus the +1 taps
+    const float lim = 1.0e6f, eps = g.box_eps;
+    const int loA = max((int)floorf(fminf(fmaxf(a0 - eps, -lim), lim)), 0);
+    const int loB = max((int)floorf(fminf(fmaxf(b0 - eps, -lim), lim)), 0);
+    const int hiA = min((int)floorf(fminf(fmaxf(a1 + eps, -lim), lim)) + 1, dims[A]);
+    const int hiB = min((int)floorf(fminf(fmaxf(b1 + eps, -lim), lim)) + 1, dims[B]);
     const int nA = hiA - loA + 1;
     const int nB = hiB - loB + 1;
     if (nA > 0 && nB > 0) {
@@ -816,12 +828,16 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     // Every sample of a ray that marches with the slabs lies inside the box
     // of its slab.  The box spans the block's slope range from the common eye
     // over the slab's extent along D (the first slab's extended back to the
-    // samples before the entry points), plus one cell on each side; a sample
-    // deviates from its exact line by the rounding of at most max_samples
-    // additions (about 1e-2 cells), so its cell never leaves the box.  Waves
-    // whose sampling lanes all march forward take the loop without a
-    // per-sample box test.
-    const bool wave_in_box = __builtin_amdgcn_ballot_w64(have && !fwd) == 0;
+    // samples before the entry points), widened by box_eps: a sample deviates
+    // from its exact line by the rounding of at most max_samples additions,
+    // which box_eps bounds, so its cell never leaves the box.  The bound
+    // holds while |pend| <= 2*max(dims): a ray's positions lie between its
+    // first sample and its exit point, so the first sample is checked.  Waves
+    // whose sampling lanes all march forward within that bound take the loop
+    // without a per-sample box test.
+    const float cmax = 2.0f * (fmaxf(g.bx, fmaxf(g.by, g.bz)) + 1.0f);
+    const bool near = fmaxf(fabsf(G.pend[0]), fmaxf(fabsf(G.pend[1]), fabsf(G.pend[2]))) <= cmax;
+    const bool wave_in_box = __builtin_amdgcn_ballot_w64(have && !(fwd && near)) == 0;
     const float maxn = (float)g.max_samples;
     float nf = (float)n;
 #pragma unroll 1
