@@ -247,6 +247,34 @@ int  stt_view_rounds(const float* start_deg, int nrounds, int views_per_round,
  * projection before every nz-th entry.  Writes np angles in degrees. */
 int  stt_lds_sequence(int np, int ns, int p, int nz, float* out_deg);
 
+/* ---- optical flow between two frames (optical_flow.h) --------------------- */
+/* Parameters of L1Huber_OF_MultiScale3D as ST_Tomography::Render calls it
+ * (ST_Tomography.cpp:441-456); stt_of_params_reference() fills those values. */
+typedef struct stt_of_params {
+    float tol;      /* residual threshold of a warp's iterations (precision) */
+    float lambda;   /* TV weight (smoothness) */
+    float huber;    /* Huber epsilon (norm 4) */
+    float eta;      /* pyramid scale per level */
+    float sigma;    /* blur of both images before the pyramid */
+    int max_iter;   /* iteration cap per warp */
+    int norm;       /* 4 = Huber-smoothed TV, otherwise TV */
+    int warps;      /* warps per level */
+    int levels;     /* pyramid levels */
+    int cut[6];     /* ROI margins at full resolution (x0 x1 y0 y1 z0 z1) */
+} stt_of_params;
+void stt_of_params_reference(stt_of_params* p);
+
+/* One pyramid level of L1Huber_OF (optical_flow.h:83-623) on host buffers:
+ * I1, I2 of dims[0]*dims[1]*dims[2] voxels (x fastest); V = 3 planes
+ * (u, v, w) and Y = 10 dual planes (y11 y12 y13 y21 y22 y23 y31 y32 y33 y4),
+ * both updated on the ROI (margins roi[6]) and kept elsewhere; ext = the
+ * world extent of the grid (aabb [0, ext]; ext = dims at full resolution).
+ * iters[w] and residual[w] (may be NULL) receive each warp's iteration count
+ * and last convergence residual. */
+int  stt_of_level(const int dims[3], const float ext[3], const float* I1,
+                  const float* I2, float* V, float* Y, const stt_of_params* p,
+                  const int roi[6], int* iters, double* residual);
+
 /* ---- host-side synthetic inputs (no device needed) ----------------------- */
 /* Additive ellipsoid phantom on an nx*ny*nz grid (seeded). */
 int  stt_synth_phantom(int nx, int ny, int nz, int n_ellipsoids, uint64_t seed,

@@ -11,6 +11,9 @@
 //   CastingRay                              (RayCaster.core.h:42-187)
 //   T_VolumeAccess trilinear + BP scatter   (SingleScalarVolume.core.h:41-146)
 //   CImg<float>::_cubic_atXYZ               (CImg.h:13429)
+//   dxp3..dzm3 finite differences           (tools.cpp:136-200, compiled as-is)
+//   CImg get_gradient / get_blur / get_resize (CImg.h:34514, 33282, 27958),
+//                                           the optical flow's image operators
 // What the reference keeps in classes that need OpenGL headers absent from
 // this image (Scene.cpp, TypicalRayProducer.cpp) is restated here as glue:
 // the per-view matrix chain of Scene::_calculateMatrix (Scene.cpp:181-207)
@@ -41,6 +44,7 @@
 #include "RayCaster.core.h"
 #include "TypicalRayProducer.core.h"
 
+#include "tools.h"
 #include "../include/stt.h"
 
 #define REF_API extern "C" __attribute__((visibility("default")))
@@ -242,4 +246,50 @@ REF_API int r_bp_ray_scatter(const stt_geometry* g, const float* q, const float*
 REF_API float r_cubic_atxyz(const float* img, int w, int h, int d, float fx, float fy, float fz) {
     CImg<float> im(img, w, h, d, 1, true);
     return im._cubic_atXYZ(fx, fy, fz, 0);
+}
+
+// tools.cpp finite differences over a whole volume: 0/1/2 = dxp3/dyp3/dzp3,
+// 3/4/5 = dxm3/dym3/dzm3.
+REF_API int r_of_diff(const int* dims, const float* a, int which, float* out) {
+    int d[3] = {dims[0], dims[1], dims[2]};
+    float* data = const_cast<float*>(a);
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                float v;
+                switch (which) {
+                    case 0: v = dxp3(data, d, i, j, k); break;
+                    case 1: v = dyp3(data, d, i, j, k); break;
+                    case 2: v = dzp3(data, d, i, j, k); break;
+                    case 3: v = dxm3(data, d, i, j, k); break;
+                    case 4: v = dym3(data, d, i, j, k); break;
+                    default: v = dzm3(data, d, i, j, k); break;
+                }
+                out[index3DtoLinear(d, i, j, k)] = v;
+            }
+    return 0;
+}
+
+REF_API int r_gradient_central(const int* dims, const float* img, float* gx, float* gy, float* gz) {
+    CImg<float> im(img, dims[0], dims[1], dims[2], 1, true);
+    CImgList<float> g = im.get_gradient("xyz", 0);
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    std::memcpy(gx, g[0]._data, n * sizeof(float));
+    std::memcpy(gy, g[1]._data, n * sizeof(float));
+    std::memcpy(gz, g[2]._data, n * sizeof(float));
+    return 0;
+}
+
+REF_API int r_blur(const int* dims, int spectrum, const float* img, float sigma, float* out) {
+    CImg<float> im(img, dims[0], dims[1], dims[2], spectrum, true);
+    CImg<float> b = im.get_blur(sigma);
+    std::memcpy(out, b._data, b.size() * sizeof(float));
+    return 0;
+}
+
+REF_API int r_resize(const int* dims, int spectrum, const float* img, int nx, int ny, int nz, int interp, float* out) {
+    CImg<float> im(img, dims[0], dims[1], dims[2], spectrum, true);
+    CImg<float> r = im.get_resize(nx, ny, nz, spectrum, interp);
+    std::memcpy(out, r._data, r.size() * sizeof(float));
+    return 0;
 }

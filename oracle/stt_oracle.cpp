@@ -456,19 +456,20 @@ template <class R> R cubic_atxyz(const R* img, int w, int h, int dd, R fx, R fy,
     return cub(dz, plane[0], plane[1], plane[2], plane[3]);
 }
 
-// warping::world_to_cell_grid for aabb = [0, dims] (warping.h:31-36): float ops, then -0.5 in double.
-template <class R> inline R to_cell(int dim, R world) {
-    R lo = R(0), hi = R(dim);
+// warping::world_to_cell_grid for aabb = [0, ext] (warping.h:31-36): float ops, then -0.5 in double.
+// The ST path uses ext = dims; the flow pyramid keeps the finest level's extent.
+template <class R> inline R to_cell(int dim, R world, R ext = R(-1)) {
+    R lo = R(0), hi = ext < R(0) ? R(dim) : ext;
     if (sizeof(R) == 4) return R(double(R(dim) * (world - lo) / (hi - lo)) - 0.5);
     return R(dim) * (world - lo) / (hi - lo) - R(0.5);
 }
 
 // warping::sample_image (warping.h:49-88): clamp to [1, dim-2], 8-corner min/max, clamped cubic.
 // interp = STT_INTERP_TRILINEAR replaces the cubic by trilinear at the same point.
-template <class R> R sample_image(const int* dims, const R* img, const R p[3], int interp) {
+template <class R> R sample_image(const int* dims, const R* img, const R p[3], int interp, const R* ext = nullptr) {
     R g[3];
     for (int c = 0; c < 3; ++c) {
-        g[c] = to_cell<R>(dims[c], p[c]);
+        g[c] = to_cell<R>(dims[c], p[c], ext ? ext[c] : R(-1));
         g[c] = std::max(R(1), std::min(R(dims[c] - 2), g[c]));
     }
     const int ix = int(g[0]), iy = int(g[1]), iz = int(g[2]);
@@ -496,7 +497,8 @@ template <class R> R sample_image(const int* dims, const R* img, const R p[3], i
 }
 
 // warping::trace (warping.h:100-114) and warp (:116-145); velocity planes u, v, w.
-template <class R> void warp(const int* dims, const R* uvw, const R* src, R dt, R* out, int interp, int nthreads) {
+template <class R>
+void warp(const int* dims, const R* uvw, const R* src, R dt, R* out, int interp, int nthreads, const R* ext = nullptr) {
     const size_t V = (size_t)dims[0] * dims[1] * dims[2];
     const R* vel[3] = {uvw, uvw + V, uvw + 2 * V};
     auto job = [&](int z) {
@@ -505,15 +507,18 @@ template <class R> void warp(const int* dims, const R* uvw, const R* src, R dt, 
                 // cell_grid_to_world (warping.h:24-29): (i + 0.5) * (hi - lo) / dim in double
                 R p[3];
                 const int ijk[3] = {x, y, z};
-                for (int c = 0; c < 3; ++c) p[c] = R(0.0 + (double(R(ijk[c])) + 0.5) * double(R(dims[c]) - R(0)) / double(R(dims[c])));
+                for (int c = 0; c < 3; ++c) {
+                    const R e = ext ? ext[c] : R(dims[c]);
+                    p[c] = R(0.0 + (double(R(ijk[c])) + 0.5) * double(e - R(0)) / double(R(dims[c])));
+                }
                 const R tdt = -dt;
                 R v1[3], p2[3], v2[3], p0[3], q[3];
-                for (int c = 0; c < 3; ++c) v1[c] = sample_image<R>(dims, vel[c], p, interp);
+                for (int c = 0; c < 3; ++c) v1[c] = sample_image<R>(dims, vel[c], p, interp, ext);
                 for (int c = 0; c < 3; ++c) p2[c] = p[c] + tdt * v1[c];
-                for (int c = 0; c < 3; ++c) v2[c] = sample_image<R>(dims, vel[c], p2, interp);
+                for (int c = 0; c < 3; ++c) v2[c] = sample_image<R>(dims, vel[c], p2, interp, ext);
                 for (int c = 0; c < 3; ++c) p0[c] = p2[c] - tdt * v2[c];
                 for (int c = 0; c < 3; ++c) q[c] = p2[c] + R(0.5) * (p[c] - p0[c]);
-                out[(size_t)x + (size_t)dims[0] * ((size_t)y + (size_t)dims[1] * z)] = sample_image<R>(dims, src, q, interp);
+                out[(size_t)x + (size_t)dims[0] * ((size_t)y + (size_t)dims[1] * z)] = sample_image<R>(dims, src, q, interp, ext);
             }
     };
     if (nthreads > 0) {  // the reference's interleaved z-slices on std::threads (warping.h:122-142)
@@ -879,6 +884,215 @@ int o_correction_f32(int n, float factor, const float* b, const float* Ax, const
 float o_cubic_atxyz_f32(const float* img, int w, int h, int d, float fx, float fy, float fz) {
     return cubic_atxyz<float>(img, w, h, d, fx, fy, fz);
 }
+int o_warp_ext_f32(const int* dims, const float* ext, const float* uvw, const float* src, float dt, float* out) {
+    warp<float>(dims, uvw, src, dt, out, STT_INTERP_CUBIC_CLAMPED, 0, ext);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// 3-D L1/Huber-TV optical flow, one pyramid level (optical_flow.h:83-623),
+// fp32 in the reference's expression order.  Two choices of this oracle:
+// the convergence sums are accumulated in double in a fixed order (the
+// reference adds them in float across OpenMP threads, one of them without a
+// reduction clause, :325-333), and the gradient planes outside the ROI are
+// zero (the reference never writes them; fresh allocations read zero).
+
+static inline size_t of_ix(const int* d, int i, int j, int k) { return (size_t)i + (size_t)d[0] * ((size_t)j + (size_t)d[1] * k); }
+// tools.cpp:136-147: forward differences, 0 on the last index
+static inline float of_dxp(const float* a, const int* d, int i, int j, int k) {
+    return i < d[0] - 1 ? a[of_ix(d, i + 1, j, k)] - a[of_ix(d, i, j, k)] : 0.0f;
+}
+static inline float of_dyp(const float* a, const int* d, int i, int j, int k) {
+    return j < d[1] - 1 ? a[of_ix(d, i, j + 1, k)] - a[of_ix(d, i, j, k)] : 0.0f;
+}
+static inline float of_dzp(const float* a, const int* d, int i, int j, int k) {
+    return k < d[2] - 1 ? a[of_ix(d, i, j, k + 1)] - a[of_ix(d, i, j, k)] : 0.0f;
+}
+// tools.cpp:153-200: backward differences; a at the first index, -a[n-2] at the last
+static inline float of_dxm(const float* a, const int* d, int i, int j, int k) {
+    if (i > 0 && i < d[0] - 1) return a[of_ix(d, i, j, k)] - a[of_ix(d, i - 1, j, k)];
+    if (i == 0) return a[of_ix(d, i, j, k)];
+    return -a[of_ix(d, i - 1, j, k)];
+}
+static inline float of_dym(const float* a, const int* d, int i, int j, int k) {
+    if (j > 0 && j < d[1] - 1) return a[of_ix(d, i, j, k)] - a[of_ix(d, i, j - 1, k)];
+    if (j == 0) return a[of_ix(d, i, j, k)];
+    return -a[of_ix(d, i, j - 1, k)];
+}
+static inline float of_dzm(const float* a, const int* d, int i, int j, int k) {
+    if (k > 0 && k < d[2] - 1) return a[of_ix(d, i, j, k)] - a[of_ix(d, i, j, k - 1)];
+    if (k == 0) return a[of_ix(d, i, j, k)];
+    return -a[of_ix(d, i, j, k - 1)];
+}
+static inline float of_min(float a, float b) { return b < a ? b : a; }  // std::min
+static inline float of_max(float a, float b) { return a < b ? b : a; }  // std::max
+
+int o_of_diff(const int* d, const float* a, int which, float* out) {
+    const int n = d[0] * d[1] * d[2];
+    (void)n;
+#pragma omp parallel for
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                float v;
+                switch (which) {
+                    case 0: v = of_dxp(a, d, i, j, k); break;
+                    case 1: v = of_dyp(a, d, i, j, k); break;
+                    case 2: v = of_dzp(a, d, i, j, k); break;
+                    case 3: v = of_dxm(a, d, i, j, k); break;
+                    case 4: v = of_dym(a, d, i, j, k); break;
+                    default: v = of_dzm(a, d, i, j, k); break;
+                }
+                out[of_ix(d, i, j, k)] = v;
+            }
+    return 0;
+}
+
+// CImg get_gradient("xyz", 0) (CImg.h:34570-34580): (I[+1] - I[-1]) / 2 with
+// neighbours clamped at the borders (cimg_for3x3x3).
+int o_gradient_central(const int* d, const float* I, float* gx, float* gy, float* gz) {
+#pragma omp parallel for
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                const int ip = std::min(i + 1, d[0] - 1), im = std::max(i - 1, 0);
+                const int jp = std::min(j + 1, d[1] - 1), jm = std::max(j - 1, 0);
+                const int kp = std::min(k + 1, d[2] - 1), km = std::max(k - 1, 0);
+                const size_t q = of_ix(d, i, j, k);
+                gx[q] = (I[of_ix(d, ip, j, k)] - I[of_ix(d, im, j, k)]) / 2;
+                gy[q] = (I[of_ix(d, i, jp, k)] - I[of_ix(d, i, jm, k)]) / 2;
+                gz[q] = (I[of_ix(d, i, j, kp)] - I[of_ix(d, i, j, km)]) / 2;
+            }
+    return 0;
+}
+
+// L1Huber_OF (optical_flow.h:83-623).  V: 3 planes (u, v, w), updated in the
+// ROI; Y: 10 planes (y11 y12 y13 y21 y22 y23 y31 y32 y33 y4), written back in
+// the ROI (:503-531).  roi = {x0, x1, y0, y1, z0, z1} margins (the global ROI[]).
+// iters_out[w] = iterations of warp w; err_out[w] = its last residual.
+int o_of_level(const int* d, const float* ext, const float* I1, const float* I2, float* V, float* Y, float tol,
+               float lambda, int max_iter, int norm, int warps, float huber, const int* roi, int* iters_out,
+               double* err_out) {
+    const size_t n = (size_t)d[0] * d[1] * d[2];
+    std::vector<float> v(3 * n), y(10 * n), Kx(10 * n, 0.0f), Kty(3 * n, 0.0f), KtyOld(3 * n, 0.0f), vOld(3 * n, 0.0f);
+    std::vector<float> ux(n, 0.0f), uy(n, 0.0f), uz(n, 0.0f), ut(n, 0.0f), tau(3 * n), sig2(n), W(n), g0(n), g1(n), g2(n);
+    std::copy(V, V + 3 * n, v.begin());
+    std::copy(Y, Y + 10 * n, y.begin());
+    const float stepsize = 1.0f;
+    const float sigma1 = of_min(stepsize / 3.0f, of_min(stepsize / 3.0f, stepsize / 3.0f));  // :170
+    const float hf = 1.0f / (1.0f + sigma1 * huber / lambda);                                 // :183
+    auto in_roi = [&](int i, int j, int k) {
+        return i >= roi[0] && i < d[0] - roi[1] && j >= roi[2] && j < d[1] - roi[3] && k >= roi[4] && k < d[2] - roi[5];
+    };
+    for (int w = 0; w < warps; ++w) {
+        // _warpI2 = warp(dim, aabb, V, image2, -dt) and its central gradient (:245-247)
+        warp<float>(d, v.data(), I2, -1.0f, W.data(), STT_INTERP_CUBIC_CLAMPED, 0, ext);
+        o_gradient_central(d, W.data(), g0.data(), g1.data(), g2.data());
+#pragma omp parallel for
+        for (int k = 0; k < d[2]; ++k)
+            for (int j = 0; j < d[1]; ++j)
+                for (int i = 0; i < d[0]; ++i) {
+                    if (!in_roi(i, j, k)) continue;
+                    const size_t q = of_ix(d, i, j, k);
+                    ux[q] = g0[q];
+                    uy[q] = g1[q];
+                    uz[q] = g2[q];
+                    ut[q] = W[q] - I1[q] - ux[q] * v[q] - uy[q] * v[n + q] - uz[q] * v[2 * n + q];  // :264
+                }
+        for (size_t q = 0; q < n; ++q) {  // :273-287
+            tau[q] = 4.0f / stepsize + std::fabs(ux[q]);
+            tau[n + q] = 4.0f / stepsize + std::fabs(uy[q]);
+            tau[2 * n + q] = 4.0f / stepsize + std::fabs(uz[q]);
+            sig2[q] = std::fabs(ux[q]) + std::fabs(uy[q]) + std::fabs(uz[q]);
+            tau[q] = 1.0f / tau[q];
+            tau[n + q] = 1.0f / tau[n + q];
+            tau[2 * n + q] = 1.0f / tau[2 * n + q];
+            sig2[q] = 1.0f / sig2[q];
+        }
+        int it = 0;
+        double err = 1.0;
+        while (err > tol && it <= max_iter) {
+            ++it;
+            const bool res = it % 50 == 0;
+            double pacc = 0.0, dacc = 0.0;
+            // primal: Kty = -div y + grad(I) y4, v -= tau Kty, on the ROI (:300-340)
+#pragma omp parallel for reduction(+ : pacc)
+            for (int k = 0; k < d[2]; ++k)
+                for (int j = 0; j < d[1]; ++j)
+                    for (int i = 0; i < d[0]; ++i) {
+                        if (!in_roi(i, j, k)) continue;
+                        const size_t q = of_ix(d, i, j, k);
+                        const float u3[3] = {ux[q], uy[q], uz[q]};
+                        float t3[3];
+                        for (int c = 0; c < 3; ++c) {
+                            if (res) {
+                                vOld[c * n + q] = v[c * n + q];
+                                KtyOld[c * n + q] = Kty[c * n + q];
+                            }
+                            const float* yc = y.data() + 3 * c * n;
+                            const float kt = -stepsize * of_dxm(yc, d, i, j, k) - stepsize * of_dym(yc + n, d, i, j, k) -
+                                             stepsize * of_dzm(yc + 2 * n, d, i, j, k) + u3[c] * y[9 * n + q];
+                            Kty[c * n + q] = kt;
+                            v[c * n + q] -= tau[c * n + q] * kt;
+                            if (res)
+                                t3[c] = std::fabs((vOld[c * n + q] - v[c * n + q]) / tau[c * n + q] - KtyOld[c * n + q] + kt);
+                        }
+                        if (res) pacc += (double)(t3[0] + t3[1] + t3[2]);
+                    }
+            // dual on every voxel (:343-480)
+#pragma omp parallel for reduction(+ : dacc)
+            for (int k = 0; k < d[2]; ++k)
+                for (int j = 0; j < d[1]; ++j)
+                    for (int i = 0; i < d[0]; ++i) {
+                        const size_t q = of_ix(d, i, j, k);
+                        float yo[10], ko[10], kn[10];
+                        for (int m = 0; m < 10; ++m) {
+                            yo[m] = y[m * n + q];
+                            ko[m] = Kx[m * n + q];
+                        }
+                        for (int c = 0; c < 3; ++c) {
+                            const float* vc = v.data() + c * n;
+                            kn[3 * c] = stepsize * of_dxp(vc, d, i, j, k);
+                            kn[3 * c + 1] = stepsize * of_dyp(vc, d, i, j, k);
+                            kn[3 * c + 2] = stepsize * of_dzp(vc, d, i, j, k);
+                        }
+                        kn[9] = ux[q] * v[q] + uy[q] * v[n + q] + uz[q] * v[2 * n + q];
+                        for (int c = 0; c < 3; ++c) {
+                            float t[3];
+                            for (int e = 0; e < 3; ++e) {
+                                const int m = 3 * c + e;
+                                t[e] = yo[m] + sigma1 * (kn[m] + kn[m] - ko[m]);
+                                if (norm == 4) t[e] = t[e] * hf;
+                            }
+                            const float div = of_max(1.0f, std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) / lambda);
+                            for (int e = 0; e < 3; ++e) y[(3 * c + e) * n + q] = t[e] / div;
+                        }
+                        y[9 * n + q] = of_max(-1.0f, of_min(1.0f, yo[9] + sig2[q] * (kn[9] + kn[9] - ko[9] + ut[q])));
+                        for (int m = 0; m < 10; ++m) Kx[m * n + q] = kn[m];
+                        if (res) {
+                            float acc = 0.0f;
+                            for (int m = 0; m < 9; ++m) acc += std::fabs((yo[m] - y[m * n + q]) / sigma1 - ko[m] + kn[m]);
+                            acc += std::fabs((yo[9] - y[9 * n + q]) / sig2[q] - ko[9] + kn[9]);
+                            dacc += (double)acc;
+                        }
+                    }
+            if (res) err = (dacc * dacc + pacc * pacc) / (double)n;
+        }
+        iters_out[w] = it;
+        err_out[w] = err;
+    }
+    // write back on the ROI (:503-531)
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) {
+                if (!in_roi(i, j, k)) continue;
+                const size_t q = of_ix(d, i, j, k);
+                for (int m = 0; m < 10; ++m) Y[m * n + q] = y[m * n + q];
+                for (int c = 0; c < 3; ++c) V[c * n + q] = v[c * n + q];
+            }
+    return 0;
+}
+
 int o_warp_f32(const int* dims, const float* uvw, const float* src, float dt, float* out, int interp, int nthreads) {
     warp<float>(dims, uvw, src, dt, out, interp, nthreads);
     return 0;

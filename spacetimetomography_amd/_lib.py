@@ -61,6 +61,13 @@ class Timing(C.Structure):
                 ("aux_ms", C.c_double), ("aux_launches", C.c_int)]
 
 
+class OfParams(C.Structure):
+    """stt_of_params (L1Huber_OF_MultiScale3D's arguments)."""
+    _fields_ = [("tol", C.c_float), ("lambda_", C.c_float), ("huber", C.c_float), ("eta", C.c_float),
+                ("sigma", C.c_float), ("max_iter", C.c_int), ("norm", C.c_int), ("warps", C.c_int),
+                ("levels", C.c_int), ("cut", C.c_int * 6)]
+
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libstt.so not built at {LIB_PATH}; run spacetimetomography_amd/build.py")
 
@@ -113,6 +120,8 @@ _SIGS = {
     "stt_synth_angles": (C.c_int, [C.c_int, C.c_int, C.c_int, _F]),
     "stt_view_rounds": (C.c_int, [_F, C.c_int, C.c_int, _F]),
     "stt_lds_sequence": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _F]),
+    "stt_of_params_reference": (None, [C.POINTER(OfParams)]),
+    "stt_of_level": (C.c_int, [_I, _F, _F, _F, _F, _F, C.POINTER(OfParams), _I, _I, C.POINTER(C.c_double)]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -200,6 +209,32 @@ def lds_sequence(np_, ns=1, p=2, nz=10**9):
     out = np.zeros(np_, np.float32)
     check(lib.stt_lds_sequence(np_, ns, p, nz, fptr(out)))
     return out
+
+
+def of_params_reference():
+    p = OfParams()
+    lib.stt_of_params_reference(C.byref(p))
+    return p
+
+
+def of_level(dims, ext, I1, I2, V, Y, params, roi=(1,) * 6):
+    """One level of L1Huber_OF on the device; returns (V, Y, iterations, residuals) per warp."""
+    dims = np.asarray(dims, np.int32)
+    ext = np.asarray(ext, np.float32)
+    roi = np.asarray(roi, np.int32)
+    n = int(np.prod(dims))
+    I1 = np.ascontiguousarray(I1, np.float32)
+    I2 = np.ascontiguousarray(I2, np.float32)
+    V = np.ascontiguousarray(V, np.float32).copy()
+    Y = np.ascontiguousarray(Y, np.float32).copy()
+    if I1.size != n or I2.size != n or V.size != 3 * n or Y.size != 10 * n:
+        raise ValueError("I1, I2 need n voxels, V 3n, Y 10n")
+    its = np.zeros(max(params.warps, 1), np.int32)
+    res = np.zeros(max(params.warps, 1), np.float64)
+    check(lib.stt_of_level(dims.ctypes.data_as(_I), fptr(ext), fptr(I1), fptr(I2), fptr(V), fptr(Y), C.byref(params),
+                           roi.ctypes.data_as(_I), its.ctypes.data_as(_I),
+                           res.ctypes.data_as(C.POINTER(C.c_double))))
+    return V, Y, its[:params.warps], res[:params.warps]
 
 
 class Context:

@@ -50,6 +50,101 @@ size_t tail_pad(const int* d) {
     return (p + 63) / 64 * 64;
 }
 
+// Owners released on every return path of the entry points that allocate.
+struct DevMem {
+    void* p = nullptr;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+};
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    ~OwnedStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+struct OwnedGraphExec {
+    hipGraphExec_t e = nullptr;
+    ~OwnedGraphExec() {
+        if (e) (void)hipGraphExecDestroy(e);
+    }
+};
+
+// ---- optical flow, one level on device buffers ----------------------------
+// V (3 planes) is updated in place on the ROI; Y (10 planes) receives the
+// dual variables on the ROI (optical_flow.h:503-531).
+int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const float* I2, float* V, float* Y,
+                    const stt_of_params& p, const int roi[6], hipStream_t s, int* iters, double* residual) {
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const int nb = of_partials(g);
+    DevMem mem;
+    // W | G(4) | T(4) | Kx(10) | Kty(3) | y(10) floats, then 2 nb + 2 doubles
+    const size_t nf = 32 * n;
+    HIPCHK(hipMalloc(&mem.p, nf * sizeof(float) + (2 * (size_t)nb + 2) * sizeof(double)));
+    float* W = (float*)mem.p;
+    float* G = W + n;
+    float* T = G + 4 * n;
+    float* Kx = T + 4 * n;
+    float* Kty = Kx + 10 * n;
+    float* y = Kty + 3 * n;
+    double* part = (double*)(W + nf);
+    double* sums = part + 2 * (size_t)nb;
+    HIPCHK(hipMemsetAsync(G, 0, 4 * n * sizeof(float), s));
+    HIPCHK(hipMemsetAsync(Kx, 0, 13 * n * sizeof(float), s));
+    HIPCHK(hipMemcpyAsync(y, Y, 10 * n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    const float third = 1.0f / 3.0f;
+    const float sigma1 = std::min(third, std::min(third, third));  // :170
+    const float hf = 1.0f / (1.0f + sigma1 * p.huber / p.lambda);  // :183
+    const int huber = p.norm == 4;
+    auto iteration = [&](int it) -> hipError_t {
+        const int mode = it % 50 == 49 ? 1 : (it % 50 == 0 ? 2 : 0);
+        hipError_t e = launch_of_primal(mode, V, y, Kty, G, T, part, g, roi, s);
+        if (e == hipSuccess) e = launch_of_dual(mode == 2, V, y, Kx, G, T, part + nb, g, sigma1, hf, p.lambda, huber, s);
+        if (e == hipSuccess && mode == 2) e = launch_sum_partials(part, nb, sums, s);
+        if (e == hipSuccess && mode == 2) e = launch_sum_partials(part + nb, nb, sums + 1, s);
+        return e;
+    };
+    // 50 iterations (the residual's period) as one graph
+    OwnedGraphExec chunk;
+    {
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int i = 1; i <= 50 && e == hipSuccess; ++i) e = iteration(i);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = ce;
+        if (e == hipSuccess) e = hipGraphInstantiate(&chunk.e, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        HIPCHK(e);
+    }
+    double host[2];
+    for (int w = 0; w < p.warps; ++w) {
+        HIPCHK(launch_of_warp(V, I2, W, g, ext, s));
+        HIPCHK(launch_of_prep(W, I1, V, G, T, g, roi, s));
+        int it = 0;
+        double err = 1.0;
+        while (err > p.tol && it <= p.max_iter) {  // :290
+            if (it % 50 == 0 && it + 50 <= p.max_iter + 1) {
+                HIPCHK(hipGraphLaunch(chunk.e, s));
+                it += 50;
+            } else {
+                ++it;
+                HIPCHK(iteration(it));
+            }
+            if (it % 50 == 0) {
+                HIPCHK(hipMemcpyAsync(host, sums, sizeof(host), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                err = (host[1] * host[1] + host[0] * host[0]) / (double)n;  // :485
+            }
+        }
+        if (iters) iters[w] = it;
+        if (residual) residual[w] = err;
+    }
+    HIPCHK(launch_of_roi_copy(Y, y, 10, g, roi, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return STT_OK;
+}
+
 }  // namespace
 
 struct stt_ctx {
@@ -1168,6 +1263,59 @@ int stt_power_iteration(stt_ctx* ctx, int iters, double* L) {
     HIPCHK(hipFreeAsync(red, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *L = std::sqrt(std::sqrt(lam2));  // lambda_max(K^T K) = ||K^T K v||; ||K|| = sqrt of it
+    return STT_OK;
+}
+
+void stt_of_params_reference(stt_of_params* p) {
+    if (!p) return;
+    // ST_Tomography.cpp:443-456; the ROI margins default to optical_flow.h:24
+    p->tol = 0.1f;
+    p->lambda = 1.2f;
+    p->huber = 1.0f;
+    p->eta = 0.65f;
+    p->sigma = 0.5f;
+    p->max_iter = 3000;
+    p->norm = 4;
+    p->warps = 3;
+    p->levels = 3;
+    for (int k = 0; k < 6; ++k) p->cut[k] = 1;
+}
+
+int stt_of_level(const int dims[3], const float ext[3], const float* I1, const float* I2, float* V, float* Y,
+                 const stt_of_params* p, const int roi[6], int* iters, double* residual) {
+    if (!dims || !ext || !I1 || !I2 || !V || !Y || !p || !roi) return fail(STT_ERR_INVALID, "null argument");
+    for (int c = 0; c < 3; ++c) {
+        if (dims[c] < 4 || dims[c] > 4096) return fail(STT_ERR_INVALID, "each dimension must be in [4, 4096]");
+        if (!(ext[c] > 0.0f)) return fail(STT_ERR_INVALID, "extent must be positive");
+    }
+    for (int k = 0; k < 6; ++k)
+        if (roi[k] < 0) return fail(STT_ERR_INVALID, "ROI margins must be >= 0");
+    if (p->warps < 0 || p->max_iter < 0 || !(p->lambda > 0.0f)) return fail(STT_ERR_INVALID, "bad flow parameters");
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    if (n * 16 * sizeof(float) >= (size_t)1 << 32) return fail(STT_ERR_INVALID, "a level must stay below 2^26 voxels");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(STT_ERR_NODEVICE, "no HIP device");
+    VolGeom g{};
+    g.nx = dims[0];
+    g.ny = dims[1];
+    g.nz = dims[2];
+    OwnedStream st;
+    HIPCHK(hipStreamCreate(&st.s));
+    DevMem buf;
+    HIPCHK(hipMalloc(&buf.p, 15 * n * sizeof(float)));
+    float* dI1 = (float*)buf.p;
+    float* dI2 = dI1 + n;
+    float* dV = dI2 + n;
+    float* dY = dV + 3 * n;
+    HIPCHK(hipMemcpyAsync(dI1, I1, n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    HIPCHK(hipMemcpyAsync(dI2, I2, n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    HIPCHK(hipMemcpyAsync(dV, V, 3 * n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    HIPCHK(hipMemcpyAsync(dY, Y, 10 * n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    const int rc = of_level_device(g, ext, dI1, dI2, dV, dY, *p, roi, st.s, iters, residual);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(V, dV, 3 * n * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipMemcpyAsync(Y, dY, 10 * n * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
     return STT_OK;
 }
 

@@ -1301,13 +1301,16 @@ struct SamplePt {
     bool xedge;         // x+2 == nx: the fourth column is clamped to nx-1
 };
 
+// world_to_cell_grid (warping.h:31-36) for aabb = [0, e]: the ST path has
+// e = dims, the flow pyramid keeps the finest level's extent on every level.
 template <bool CUBIC>
-__device__ __forceinline__ void sample_point(const VolGeom& g, float px, float py, float pz, SamplePt& s) {
+__device__ __forceinline__ void sample_point_e(const VolGeom& g, const float e[3], float px, float py, float pz,
+                                               SamplePt& s) {
     const float d[3] = {(float)g.nx, (float)g.ny, (float)g.nz};
     const float p[3] = {px, py, pz};
     float gc[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) gc[c] = __builtin_amdgcn_fmed3f((d[c] * p[c]) / d[c] - 0.5f, 1.0f, d[c] - 2.0f);
+    for (int c = 0; c < 3; ++c) gc[c] = __builtin_amdgcn_fmed3f((d[c] * p[c]) / e[c] - 0.5f, 1.0f, d[c] - 2.0f);
     const int ix = (int)gc[0], iy = (int)gc[1], iz = (int)gc[2];
     s.fx = gc[0] - (float)ix;
     s.fy = gc[1] - (float)iy;
@@ -1333,6 +1336,12 @@ __device__ __forceinline__ void sample_point(const VolGeom& g, float px, float p
             s.zoff[k] = __umul24((unsigned)(iz + k), nz4);
         }
     }
+}
+
+template <bool CUBIC>
+__device__ __forceinline__ void sample_point(const VolGeom& g, float px, float py, float pz, SamplePt& s) {
+    const float d[3] = {(float)g.nx, (float)g.ny, (float)g.nz};
+    sample_point_e<CUBIC>(g, d, px, py, pz, s);
 }
 
 // sample_image (warping.h:49-88) on channel plane `plane_off` (bytes) of an
@@ -1765,6 +1774,205 @@ __global__ __launch_bounds__(256) void k_warp_vjp(const float* __restrict__ uvw,
 // ---------------------------------------------------------------------------
 // Launchers.
 
+// ---------------------------------------------------------------------------
+// 3-D L1/Huber-TV optical flow, one pyramid level (optical_flow.h:83-623).
+// Planes of n voxels: V (3: u v w), Y (10: y11 y12 y13 y21 y22 y23 y31 y32
+// y33 y4), Kx (10: the previous iteration's K V), Kty (3, kept only where a
+// residual needs it), G (4: ux uy uz ut), T (4: tau1 tau2 tau3 sigma2).  fp32
+// in the reference's expression order; residual sums in FP64, fixed order.
+
+// warp(dim, aabb, V, I2, -dt) with dt = 1 and aabb = [0, ext] (:245): trace
+// from the cell centre in world units, clamped-cubic samples.
+__global__ __launch_bounds__(256) void k_of_warp(const float* __restrict__ v, const float* __restrict__ img,
+                                                 float* __restrict__ out, VolGeom g, float ex, float ey, float ez) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const unsigned V = (unsigned)(g.nx * g.ny * g.nz);
+    const __amdgpu_buffer_rsrc_t rs_vel = __builtin_amdgcn_make_buffer_rsrc((void*)v, (short)0, (int)(12u * V), kBufWord3);
+    const __amdgpu_buffer_rsrc_t rs_img = __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, (int)(4u * V), kBufWord3);
+    const float e[3] = {ex, ey, ez};
+    const int ijk[3] = {x, y, z};
+    const int dd[3] = {g.nx, g.ny, g.nz};
+    float p[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)  // cell_grid_to_world (warping.h:24-29), evaluated in double
+        p[c] = (float)(0.0 + ((double)(float)ijk[c] + 0.5) * (double)(e[c] - 0.0f) / (double)(float)dd[c]);
+    SamplePt s;
+    sample_point_e<true>(g, e, p[0], p[1], p[2], s);
+    float p2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p2[c] = p[c] + 1.0f * sample_img<true>(rs_vel, 4u * V * (unsigned)c, s);
+    sample_point_e<true>(g, e, p2[0], p2[1], p2[2], s);
+    float q[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float p0 = p2[c] - 1.0f * sample_img<true>(rs_vel, 4u * V * (unsigned)c, s);
+        q[c] = p2[c] + 0.5f * (p[c] - p0);
+    }
+    sample_point_e<true>(g, e, q[0], q[1], q[2], s);
+    out[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = sample_img<true>(rs_img, 0u, s);
+}
+
+struct OfRoi {
+    int x0, x1, y0, y1, z0, z1;  // margins: the ROI is [x0, nx - x1) x [y0, ny - y1) x [z0, nz - z1)
+};
+__device__ __forceinline__ bool of_in_roi(const VolGeom& g, const OfRoi& r, int x, int y, int z) {
+    return x >= r.x0 && x < g.nx - r.x1 && y >= r.y0 && y < g.ny - r.y1 && z >= r.z0 && z < g.nz - r.z1;
+}
+
+// Gradient of the warped image (CImg central differences, clamped
+// neighbours, CImg.h:34570-34580) and the linearised data term on the ROI
+// (:254-268); step sizes on every voxel (:273-287).
+__global__ __launch_bounds__(256) void k_of_prep(const float* __restrict__ W, const float* __restrict__ I1,
+                                                 const float* __restrict__ v, float* __restrict__ G,
+                                                 float* __restrict__ T, VolGeom g, OfRoi roi) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t q = (size_t)x + sy * y + sz * z;
+    float ux = G[q], uy = G[n + q], uz = G[2 * n + q];
+    if (of_in_roi(g, roi, x, y, z)) {
+        const size_t xp = x + 1 < g.nx ? q + 1 : q, xm = x > 0 ? q - 1 : q;
+        const size_t yp = y + 1 < g.ny ? q + sy : q, ym = y > 0 ? q - sy : q;
+        const size_t zp = z + 1 < g.nz ? q + sz : q, zm = z > 0 ? q - sz : q;
+        ux = (W[xp] - W[xm]) / 2.0f;
+        uy = (W[yp] - W[ym]) / 2.0f;
+        uz = (W[zp] - W[zm]) / 2.0f;
+        G[q] = ux;
+        G[n + q] = uy;
+        G[2 * n + q] = uz;
+        G[3 * n + q] = W[q] - I1[q] - ux * v[q] - uy * v[n + q] - uz * v[2 * n + q];
+    }
+    const float s = 4.0f / 1.0f;
+    T[q] = 1.0f / (s + fabsf(ux));
+    T[n + q] = 1.0f / (s + fabsf(uy));
+    T[2 * n + q] = 1.0f / (s + fabsf(uz));
+    T[3 * n + q] = 1.0f / (fabsf(ux) + fabsf(uy) + fabsf(uz));
+}
+
+// tools.cpp differences (forward: 0 on the last index; backward: a at the
+// first index, -a[n-2] at the last).
+__device__ __forceinline__ float of_dm(const float* a, size_t q, size_t st, int i, int n) {
+    if (i > 0 && i < n - 1) return a[q] - a[q - st];
+    if (i == 0) return a[q];
+    return -a[q - st];
+}
+__device__ __forceinline__ float of_dp(const float* a, size_t q, size_t st, int i, int n) {
+    return i < n - 1 ? a[q + st] - a[q] : 0.0f;
+}
+
+__device__ __forceinline__ void of_block_partial(double acc, double* partial) {
+    __shared__ double red[4];
+    const double t = block_sum_f64(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
+}
+
+// Primal step on the ROI (:300-340).  MODE 1 keeps K^T y for the next
+// iteration's residual; MODE 2 adds this iteration's primal residual.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_of_primal(float* __restrict__ v, const float* __restrict__ Y,
+                                                   float* __restrict__ kty, const float* __restrict__ G,
+                                                   const float* __restrict__ T, double* __restrict__ partial,
+                                                   VolGeom g, OfRoi roi) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    double acc = 0.0;
+    if (x < g.nx && y < g.ny && of_in_roi(g, roi, x, y, z)) {
+        const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
+        const size_t q = (size_t)x + sy * y + sz * z;
+        const float y4 = Y[9 * n + q];
+        float r = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float* yc = Y + (size_t)(3 * c) * n;
+            const float kt = -1.0f * of_dm(yc, q, 1, x, g.nx) - 1.0f * of_dm(yc + n, q, sy, y, g.ny) -
+                             1.0f * of_dm(yc + 2 * n, q, sz, z, g.nz) + G[c * n + q] * y4;
+            const float tc = T[c * n + q];
+            const float vo = v[c * n + q];
+            const float vn = vo - tc * kt;
+            v[c * n + q] = vn;
+            if (MODE == 1) kty[c * n + q] = kt;
+            if (MODE == 2) r = r + fabsf((vo - vn) / tc - kty[c * n + q] + kt);
+        }
+        acc = (double)r;
+    }
+    if (MODE == 2) of_block_partial(acc, partial);
+}
+
+// Dual step on every voxel (:343-480); RES adds the dual residual.
+template <bool RES>
+__global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, float* __restrict__ Y,
+                                                 float* __restrict__ Kx, const float* __restrict__ G,
+                                                 const float* __restrict__ T, double* __restrict__ partial,
+                                                 VolGeom g, float sigma1, float hf, float lambda, int huber) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    double acc = 0.0;
+    if (x < g.nx && y < g.ny) {
+        const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
+        const size_t q = (size_t)x + sy * y + sz * z;
+        float r = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float* vc = v + (size_t)c * n;
+            const float kn[3] = {1.0f * of_dp(vc, q, 1, x, g.nx), 1.0f * of_dp(vc, q, sy, y, g.ny),
+                                 1.0f * of_dp(vc, q, sz, z, g.nz)};
+            float t[3], yo[3], ko[3];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const size_t m = (size_t)(3 * c + e) * n + q;
+                yo[e] = Y[m];
+                ko[e] = Kx[m];
+                t[e] = yo[e] + sigma1 * (kn[e] + kn[e] - ko[e]);
+                if (huber) t[e] = t[e] * hf;
+            }
+            const float div = fmaxf(1.0f, sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) / lambda);
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const size_t m = (size_t)(3 * c + e) * n + q;
+                const float yn = t[e] / div;
+                Y[m] = yn;
+                Kx[m] = kn[e];
+                if (RES) r = r + fabsf((yo[e] - yn) / sigma1 - ko[e] + kn[e]);
+            }
+        }
+        const float k4 = G[q] * v[q] + G[n + q] * v[n + q] + G[2 * n + q] * v[2 * n + q];
+        const float s2 = T[3 * n + q];
+        const float y4o = Y[9 * n + q], k4o = Kx[9 * n + q];
+        // std::max(-1, std::min(1, a)): min(1, a) keeps 1 unless a < 1, so NaN gives 1
+        const float a = y4o + s2 * (k4 + k4 - k4o + G[3 * n + q]);
+        const float mn = a < 1.0f ? a : 1.0f;
+        const float y4n = -1.0f < mn ? mn : -1.0f;
+        Y[9 * n + q] = y4n;
+        Kx[9 * n + q] = k4;
+        if (RES) r = r + fabsf((y4o - y4n) / s2 - k4o + k4);
+        acc = (double)r;
+    }
+    if (RES) of_block_partial(acc, partial);
+}
+
+// dst = src on the ROI, for `planes` planes (the write-back of :503-531).
+__global__ __launch_bounds__(256) void k_of_roi_copy(float* __restrict__ dst, const float* __restrict__ src, int planes,
+                                                     VolGeom g, OfRoi roi) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || !of_in_roi(g, roi, x, y, z)) return;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+    for (int m = 0; m < planes; ++m) dst[(size_t)m * n + q] = src[(size_t)m * n + q];
+}
+
 static inline dim3 vol_grid(const VolGeom& g, int nitems) {
     return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nitems);
 }
@@ -1897,6 +2105,53 @@ hipError_t launch_warp(const float* uvw, const float* src, float* out, const Vol
                        hipStream_t s) {
     // warp(..., dt) traces with -dt (warping.h:126)
     hipLaunchKernelGGL(k_warp, vol_grid(g, 1), dim3(256), 0, s, uvw, src, out, g, -dt, cubic);
+    return hipGetLastError();
+}
+
+
+// ---- optical flow launchers ------------------------------------------------
+
+int of_partials(const VolGeom& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
+
+static inline OfRoi of_roi(const int* r) { return OfRoi{r[0], r[1], r[2], r[3], r[4], r[5]}; }
+
+hipError_t launch_of_warp(const float* v, const float* img, float* out, const VolGeom& g, const float* ext,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_of_warp, vol_grid(g, 1), dim3(256), 0, s, v, img, out, g, ext[0], ext[1], ext[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, float* T, const VolGeom& g,
+                          const int* roi, hipStream_t s) {
+    hipLaunchKernelGGL(k_of_prep, vol_grid(g, 1), dim3(256), 0, s, W, I1, v, G, T, g, of_roi(roi));
+    return hipGetLastError();
+}
+
+hipError_t launch_of_primal(int mode, float* v, const float* Y, float* kty, const float* G, const float* T,
+                            double* partial, const VolGeom& g, const int* roi, hipStream_t s) {
+    const dim3 grid = vol_grid(g, 1);
+    switch (mode) {
+        case 1: hipLaunchKernelGGL(k_of_primal<1>, grid, dim3(256), 0, s, v, Y, kty, G, T, partial, g, of_roi(roi)); break;
+        case 2: hipLaunchKernelGGL(k_of_primal<2>, grid, dim3(256), 0, s, v, Y, kty, G, T, partial, g, of_roi(roi)); break;
+        default: hipLaunchKernelGGL(k_of_primal<0>, grid, dim3(256), 0, s, v, Y, kty, G, T, partial, g, of_roi(roi)); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_of_dual(bool res, const float* v, float* Y, float* Kx, const float* G, const float* T,
+                          double* partial, const VolGeom& g, float sigma1, float hf, float lambda, int huber,
+                          hipStream_t s) {
+    const dim3 grid = vol_grid(g, 1);
+    if (res)
+        hipLaunchKernelGGL(k_of_dual<true>, grid, dim3(256), 0, s, v, Y, Kx, G, T, partial, g, sigma1, hf, lambda, huber);
+    else
+        hipLaunchKernelGGL(k_of_dual<false>, grid, dim3(256), 0, s, v, Y, Kx, G, T, partial, g, sigma1, hf, lambda, huber);
+    return hipGetLastError();
+}
+
+hipError_t launch_of_roi_copy(float* dst, const float* src, int planes, const VolGeom& g, const int* roi,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_of_roi_copy, vol_grid(g, 1), dim3(256), 0, s, dst, src, planes, g, of_roi(roi));
     return hipGetLastError();
 }
 

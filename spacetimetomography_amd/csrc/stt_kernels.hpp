@@ -43,4 +43,21 @@ hipError_t launch_fill_hash(float* x, size_t n, unsigned seed, hipStream_t s);
 hipError_t launch_pi_apply_k(const float* v, float* y, int F, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_pi_apply_kt(const float* y, float* w, int F, const VolGeom& g, hipStream_t s);
 
+// Optical flow, one pyramid level: aabb-aware clamped-cubic warp of I2 by V,
+// gradient + data term + step sizes, primal (mode 0 plain, 1 keep K^T y,
+// 2 residual) and dual (res: residual) steps, ROI write-back.  Residual
+// kernels write one FP64 partial per block (of_partials(g) blocks).
+int of_partials(const VolGeom& g);
+hipError_t launch_of_warp(const float* v, const float* img, float* out, const VolGeom& g, const float* ext,
+                          hipStream_t s);
+hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, float* T, const VolGeom& g,
+                          const int* roi, hipStream_t s);
+hipError_t launch_of_primal(int mode, float* v, const float* Y, float* kty, const float* G, const float* T,
+                            double* partial, const VolGeom& g, const int* roi, hipStream_t s);
+hipError_t launch_of_dual(bool res, const float* v, float* Y, float* Kx, const float* G, const float* T,
+                          double* partial, const VolGeom& g, float sigma1, float hf, float lambda, int huber,
+                          hipStream_t s);
+hipError_t launch_of_roi_copy(float* dst, const float* src, int planes, const VolGeom& g, const int* roi,
+                              hipStream_t s);
+
 }  // namespace stt

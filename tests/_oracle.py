@@ -56,6 +56,11 @@ _sigs = {
     "o_ctx_pd_extrapolate": (C.c_int, [_P, C.c_int]),
     "o_ctx_iterate": (C.c_int, [_P, C.c_int, C.c_int]),
     "o_ctx_set_per_frame_raylen": (None, [_P, C.c_int]),
+    "o_warp_ext_f32": (C.c_int, [_I, _F, _F, _F, C.c_float, _F]),
+    "o_of_diff": (C.c_int, [_I, _F, C.c_int, _F]),
+    "o_gradient_central": (C.c_int, [_I, _F, _F, _F, _F]),
+    "o_of_level": (C.c_int, [_I, _F, _F, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_float,
+                             _I, _I, _D]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(orc, _n)
@@ -73,6 +78,10 @@ if os.path.exists(REF_PATH):
         "r_fp": (C.c_int, [_G, _F, _F, _F, _F, _F, _F, _F]),
         "r_bp_ray_scatter": (C.c_int, [_G, _F, _F, _F, _F]),
         "r_cubic_atxyz": (C.c_float, [_F, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "r_of_diff": (C.c_int, [_I, _F, C.c_int, _F]),
+        "r_gradient_central": (C.c_int, [_I, _F, _F, _F, _F]),
+        "r_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
+        "r_resize": (C.c_int, [_I, C.c_int, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]),
     }.items():
         _f = getattr(ref, _n)
         _f.restype = _r
@@ -243,3 +252,36 @@ class OracleCtx:
 
     def per_frame_raylen(self, on=True):
         orc.o_ctx_set_per_frame_raylen(self.h, int(on))
+
+
+def warp_ext(dims, ext, uvw, src, dt):
+    """warping::warp with aabb = [0, ext] (the flow pyramid's coarse levels)."""
+    out = np.zeros(int(np.prod(dims)), np.float32)
+    orc.o_warp_ext_f32(i32(np.asarray(dims, np.int32)), f32(np.asarray(ext, np.float32)), f32(uvw), f32(src), float(dt),
+                       f32(out))
+    return out
+
+
+def of_diff(dims, a, which):
+    """tools.cpp differences: 0/1/2 = forward x/y/z, 3/4/5 = backward x/y/z."""
+    out = np.zeros(int(np.prod(dims)), np.float32)
+    orc.o_of_diff(i32(np.asarray(dims, np.int32)), f32(a), int(which), f32(out))
+    return out
+
+
+def gradient_central(dims, img):
+    g = [np.zeros(int(np.prod(dims)), np.float32) for _ in range(3)]
+    orc.o_gradient_central(i32(np.asarray(dims, np.int32)), f32(img), f32(g[0]), f32(g[1]), f32(g[2]))
+    return g
+
+
+def of_level(dims, ext, I1, I2, V, Y, tol=0.1, lam=1.2, max_iter=3000, norm=4, warps=3, huber=1.0, roi=(1,) * 6):
+    """L1Huber_OF on one level; returns (V, Y, iterations per warp, residual per warp)."""
+    V = np.ascontiguousarray(V, np.float32).copy()
+    Y = np.ascontiguousarray(Y, np.float32).copy()
+    its = np.zeros(warps, np.int32)
+    errs = np.zeros(warps, np.float64)
+    orc.o_of_level(i32(np.asarray(dims, np.int32)), f32(np.asarray(ext, np.float32)), f32(I1), f32(I2), f32(V), f32(Y),
+                   float(tol), float(lam), int(max_iter), int(norm), int(warps), float(huber),
+                   i32(np.asarray(roi, np.int32)), i32(its), f64(errs))
+    return V, Y, its, errs
