@@ -137,12 +137,14 @@ int  stt_get_volume_dev(stt_ctx* ctx, int frame, float* vol_dev);
 
 /* Dual / auxiliary state readout (X_bar, X_prev, Y_tp, Y_bc: one volume each;
  * Y_tv: four planes, the three spatial components then STV's temporal
- * component, which stays zero for the other priors). */
+ * component, which stays zero for the other priors; FLOW: the three planes
+ * of flow pair `frame`, 0 <= frame < nframes). */
 #define STT_BUF_XBAR  0
 #define STT_BUF_XPREV 1
 #define STT_BUF_YTV   2
 #define STT_BUF_YTP   3
 #define STT_BUF_YBC   4
+#define STT_BUF_FLOW  5
 int  stt_get_buffer(stt_ctx* ctx, int which, int frame, float* out);
 int  stt_set_buffer(stt_ctx* ctx, int which, int frame, const float* in);
 
@@ -274,6 +276,28 @@ void stt_of_params_reference(stt_of_params* p);
 int  stt_of_level(const int dims[3], const float ext[3], const float* I1,
                   const float* I2, float* V, float* Y, const stt_of_params* p,
                   const int roi[6], int* iters, double* residual);
+
+/* L1Huber_OF_MultiScale3D (optical_flow.h:630-734) on host buffers: blur
+ * both images, pyramid of p->levels levels (sizes int(dims * eta^k)),
+ * coarse to fine, V upsampled between levels; Y starts at zero as Render
+ * passes it.  V (3 planes, full resolution, voxel units) in/out.  iters
+ * (levels * warps, coarsest level first; may be NULL) receives the
+ * iteration counts.  Replaces the flow sub-optimisation call of Render
+ * (ST_Tomography.cpp:452-455). */
+int  stt_of_multiscale(const int dims[3], const float* I1, const float* I2,
+                       float* V, const stt_of_params* p, int* iters);
+/* The same on the context's device-resident frames: I1 = volume `pair`,
+ * I2 = volume `pair + 1`, V = flow `pair` (updated in place). */
+int  stt_optical_flow(stt_ctx* ctx, int pair, const stt_of_params* p,
+                      int* iters);
+/* CImg get_blur(sigma) (CImg.h:33282: order-0 Deriche along x, y, z, Neumann
+ * boundary) and get_resize(new_dims, spectrum, 5) (CImg.h:27958: moving
+ * average on shrinking axes, cubic on growing ones), on host buffers of
+ * `spectrum` planes. */
+int  stt_blur(const int dims[3], int spectrum, const float* img, float sigma,
+              float* out);
+int  stt_resize(const int dims[3], int spectrum, const float* img,
+                const int new_dims[3], float* out);
 
 /* ---- host-side synthetic inputs (no device needed) ----------------------- */
 /* Additive ellipsoid phantom on an nx*ny*nz grid (seeded). */

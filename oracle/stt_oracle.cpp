@@ -12,6 +12,7 @@
 #include "stt_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1090,6 +1091,216 @@ int o_of_level(const int* d, const float* ext, const float* I1, const float* I2,
                 for (int m = 0; m < 10; ++m) Y[m * n + q] = y[m * n + q];
                 for (int c = 0; c < 3; ++c) V[c * n + q] = v[c * n + q];
             }
+    return 0;
+}
+
+// ---- the flow pyramid's image operators (CImg 1.8.0) ----------------------
+// Deriche coefficients of deriche(sigma, 0, axis) (CImg.h:32917-32968).
+struct DericheCoef {
+    float a0, a1, a2, a3, b1, b2, coefp, coefn;
+};
+static DericheCoef deriche_coef(float sigma) {
+    const float nsigma = sigma >= 0 ? sigma : -sigma;  // (percent form not used)
+    const float nnsigma = nsigma < 0.1f ? 0.1f : nsigma, alpha = 1.695f / nnsigma, ema = (float)std::exp(-alpha),
+                ema2 = (float)std::exp(-2 * alpha), b1 = -2 * ema, b2 = ema2;
+    const float k = (1 - ema) * (1 - ema) / (1 + 2 * alpha * ema - ema2);
+    DericheCoef c;
+    c.a0 = k;
+    c.a1 = k * (alpha - 1) * ema;
+    c.a2 = k * (alpha + 1) * ema;
+    c.a3 = -k * ema2;
+    c.b1 = b1;
+    c.b2 = b2;
+    c.coefp = (c.a0 + c.a1) / (1 + b1 + b2);
+    c.coefn = (c.a2 + c.a3) / (1 + b1 + b2);
+    return c;
+}
+// One line of _cimg_deriche_apply (Neumann boundary), in place.
+static void deriche_line(float* x, long N, long off, const DericheCoef& c, std::vector<float>& Y) {
+    Y.resize(N);
+    float* px = x;
+    float yb = 0, yp = 0, xp = *px;
+    yb = yp = c.coefp * xp;
+    for (long m = 0; m < N; ++m) {
+        const float xc = *px;
+        px += off;
+        const float yc = Y[m] = c.a0 * xc + c.a1 * xp - c.b1 * yp - c.b2 * yb;
+        xp = xc;
+        yb = yp;
+        yp = yc;
+    }
+    float xn = *(px - off), xa = xn, yn = c.coefn * xn, ya = yn;
+    for (long n = N - 1; n >= 0; --n) {
+        const float xc = *(px -= off);
+        const float yc = c.a2 * xn + c.a3 * xa - c.b1 * yn - c.b2 * ya;
+        xa = xn;
+        xn = xc;
+        ya = yn;
+        yn = yc;
+        *px = Y[n] + yc;
+    }
+}
+// get_blur(sigma) (CImg.h:33249-33284): order-0 Deriche along x, y, z.
+int o_blur(const int* d, int spectrum, const float* img, float sigma, float* out) {
+    const size_t n = (size_t)d[0] * d[1] * d[2];
+    std::copy(img, img + n * spectrum, out);
+    if (sigma < 0.1f) return 0;
+    const DericheCoef c = deriche_coef(sigma);
+    for (int ch = 0; ch < spectrum; ++ch) {
+        float* im = out + ch * n;
+        std::vector<float> Y;
+        if (d[0] > 1)
+            for (int z = 0; z < d[2]; ++z)
+                for (int y = 0; y < d[1]; ++y) deriche_line(im + of_ix(d, 0, y, z), d[0], 1, c, Y);
+        if (d[1] > 1)
+            for (int z = 0; z < d[2]; ++z)
+                for (int x = 0; x < d[0]; ++x) deriche_line(im + of_ix(d, x, 0, z), d[1], d[0], c, Y);
+        if (d[2] > 1)
+            for (int y = 0; y < d[1]; ++y)
+                for (int x = 0; x < d[0]; ++x) deriche_line(im + of_ix(d, x, y, 0), d[2], (long)d[0] * d[1], c, Y);
+    }
+    return 0;
+}
+
+// One axis of get_resize(..., interpolation 5), boundary 0 (CImg.h:28432-28520):
+// moving average (type 2, CImg.h:28135-28150) when shrinking, cubic when growing.
+static void resize_axis(const std::vector<float>& src, const int* d, int axis, int ns, std::vector<float>& dst,
+                        int* dn) {
+    for (int a = 0; a < 3; ++a) dn[a] = d[a];
+    dn[axis] = ns;
+    const int w = d[axis];
+    const size_t st_s = axis == 0 ? 1 : (axis == 1 ? (size_t)d[0] : (size_t)d[0] * d[1]);
+    const size_t st_d = axis == 0 ? 1 : (axis == 1 ? (size_t)dn[0] : (size_t)dn[0] * dn[1]);
+    const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;  // the two other axes
+    dst.assign((size_t)dn[0] * dn[1] * dn[2], 0.0f);
+    auto base_s = [&](int u, int v) {
+        int c[3] = {0, 0, 0};
+        c[o1] = u;
+        c[o2] = v;
+        return of_ix(d, c[0], c[1], c[2]);
+    };
+    auto base_d = [&](int u, int v) {
+        int c[3] = {0, 0, 0};
+        c[o1] = u;
+        c[o2] = v;
+        return of_ix(dn, c[0], c[1], c[2]);
+    };
+    if (w > ns) {  // moving average
+        for (int v = 0; v < d[o2]; ++v)
+            for (int u = 0; u < d[o1]; ++u) {
+                const float* ps = src.data() + base_s(u, v);
+                float* pd = dst.data() + base_d(u, v);
+                unsigned a = (unsigned)w * ns, b = w, c = ns, s = 0, t = 0;
+                while (a) {
+                    const unsigned dd = std::min(b, c);
+                    a -= dd;
+                    b -= dd;
+                    c -= dd;
+                    pd[t * st_d] += ps[s * st_s] * dd;
+                    if (!b) {
+                        pd[t * st_d] /= w;
+                        ++t;
+                        b = w;
+                    }
+                    if (!c) {
+                        ++s;
+                        c = ns;
+                    }
+                }
+            }
+        return;
+    }
+    // cubic: source index and fraction per output from CImg's double recurrence
+    const double f = ns > 1 ? (w - 1.0) / (ns - 1) : 0;
+    std::vector<unsigned> sidx(ns);
+    std::vector<double> frac(ns);
+    double curr = 0;
+    for (int x = 0; x < ns; ++x) {
+        sidx[x] = (unsigned)curr;
+        frac[x] = curr - (unsigned)curr;
+        curr = std::min(w - 1.0, curr + f);
+    }
+    for (int v = 0; v < d[o2]; ++v)
+        for (int u = 0; u < d[o1]; ++u) {
+            const float* ps = src.data() + base_s(u, v);
+            float* pd = dst.data() + base_d(u, v);
+            for (int x = 0; x < ns; ++x) {
+                const long si = sidx[x];
+                const double t = frac[x], val1 = ps[si * st_s], val0 = si > 0 ? (double)ps[(si - 1) * st_s] : val1,
+                             val2 = si <= w - 2 ? (double)ps[(si + 1) * st_s] : val1,
+                             val3 = si < w - 2 ? (double)ps[(si + 2) * st_s] : val2,
+                             val = val1 + 0.5f * (t * (-val0 + val2) + t * t * (2 * val0 - 5 * val1 + 4 * val2 - val3) +
+                                                  t * t * t * (-val0 + 3 * val1 - 3 * val2 + val3));
+                pd[x * st_d] = (float)val;
+            }
+        }
+}
+int o_resize(const int* d, int spectrum, const float* img, const int* nd, float* out) {
+    const size_t n = (size_t)d[0] * d[1] * d[2], nn = (size_t)nd[0] * nd[1] * nd[2];
+    for (int ch = 0; ch < spectrum; ++ch) {
+        std::vector<float> cur(img + ch * n, img + (ch + 1) * n), nxt;
+        int cd[3] = {d[0], d[1], d[2]}, ndim[3];
+        for (int a = 0; a < 3; ++a) {
+            if (nd[a] == cd[a]) continue;
+            resize_axis(cur, cd, a, nd[a], nxt, ndim);
+            cur.swap(nxt);
+            for (int k = 0; k < 3; ++k) cd[k] = ndim[k];
+        }
+        std::copy(cur.begin(), cur.end(), out + ch * nn);
+    }
+    return 0;
+}
+
+// L1Huber_OF_MultiScale3D (optical_flow.h:630-734) with Y1..Y4 = 0 as Render
+// passes them (ST_Tomography.cpp:452-455).  V: 3 planes in/out at full size.
+// level_dims (levels x 3) and iters (levels x warps) may be null.
+int o_of_multiscale(const int* D, const float* I1, const float* I2, float* V, float tol, float lambda, int max_iter,
+                    int norm, int warps, float eta, float sigma, int levels, float huber, const int* cut,
+                    int* level_dims, int* iters) {
+    const size_t N = (size_t)D[0] * D[1] * D[2];
+    std::vector<float> b1(N), b2(N);
+    o_blur(D, 1, I1, sigma, b1.data());
+    o_blur(D, 1, I2, sigma, b2.data());
+    std::vector<std::array<int, 3>> dims(levels);
+    std::vector<std::vector<float>> im1(levels), im2(levels), vs(levels);
+    float t_eta = 1.0f;
+    for (int s = 0; s < levels; ++s) {  // finest first (:660-676)
+        std::array<int, 3> nd;
+        for (int a = 0; a < 3; ++a) nd[a] = (int)((float)D[a] * t_eta);
+        dims[s] = nd;
+        const size_t m = (size_t)nd[0] * nd[1] * nd[2];
+        im1[s].resize(m);
+        im2[s].resize(m);
+        vs[s].resize(3 * m);
+        o_resize(D, 1, b1.data(), nd.data(), im1[s].data());
+        o_resize(D, 1, b2.data(), nd.data(), im2[s].data());
+        o_resize(D, 3, V, nd.data(), vs[s].data());
+        t_eta = t_eta * eta;
+    }
+    const float ext[3] = {(float)D[0], (float)D[1], (float)D[2]};
+    std::vector<float> Y;  // Y1..Y4 = 0, resized zeros stay zero
+    for (int k = 0; k < levels; ++k) {  // coarsest first (:690-729)
+        const int s = levels - 1 - k;
+        const int* nd = dims[s].data();
+        const size_t m = (size_t)nd[0] * nd[1] * nd[2];
+        if (k == 0) Y.assign(10 * m, 0.0f);
+        int roi[6];
+        for (int a = 0; a < 6; ++a) roi[a] = (int)((float)cut[a] * ((float)nd[a / 2] / (float)D[a / 2]));  // :700-705
+        std::vector<double> errs(warps);
+        o_of_level(nd, ext, im1[s].data(), im2[s].data(), vs[s].data(), Y.data(), tol, lambda, max_iter, norm, warps,
+                   huber, roi, iters ? iters + k * warps : std::vector<int>(warps).data(), errs.data());
+        if (level_dims)
+            for (int a = 0; a < 3; ++a) level_dims[3 * k + a] = nd[a];
+        if (k + 1 < levels) {  // :712-722
+            const int* nn = dims[s - 1].data();
+            const size_t mn = (size_t)nn[0] * nn[1] * nn[2];
+            std::vector<float> Yn(10 * mn);
+            o_resize(nd, 3, vs[s].data(), nn, vs[s - 1].data());
+            o_resize(nd, 10, Y.data(), nn, Yn.data());
+            Y.swap(Yn);
+        }
+    }
+    std::copy(vs[0].begin(), vs[0].end(), V);
     return 0;
 }
 

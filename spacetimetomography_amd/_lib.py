@@ -22,7 +22,7 @@ STT_ERR_NODEVICE = 5
 STT_BP_VOXEL, STT_BP_RAY = 0, 1
 STT_PRIOR_ATV, STT_PRIOR_TV, STT_PRIOR_STV = 0, 1, 2
 STT_INTERP_CUBIC_CLAMPED, STT_INTERP_TRILINEAR = 0, 1
-STT_BUF_XBAR, STT_BUF_XPREV, STT_BUF_YTV, STT_BUF_YTP, STT_BUF_YBC = range(5)
+STT_BUF_XBAR, STT_BUF_XPREV, STT_BUF_YTV, STT_BUF_YTP, STT_BUF_YBC, STT_BUF_FLOW = range(6)
 
 
 class Geometry(C.Structure):
@@ -122,6 +122,10 @@ _SIGS = {
     "stt_lds_sequence": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _F]),
     "stt_of_params_reference": (None, [C.POINTER(OfParams)]),
     "stt_of_level": (C.c_int, [_I, _F, _F, _F, _F, _F, C.POINTER(OfParams), _I, _I, C.POINTER(C.c_double)]),
+    "stt_of_multiscale": (C.c_int, [_I, _F, _F, _F, C.POINTER(OfParams), _I]),
+    "stt_optical_flow": (C.c_int, [_P, C.c_int, C.POINTER(OfParams), _I]),
+    "stt_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
+    "stt_resize": (C.c_int, [_I, C.c_int, _F, _I, _F]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -237,6 +241,38 @@ def of_level(dims, ext, I1, I2, V, Y, params, roi=(1,) * 6):
     return V, Y, its[:params.warps], res[:params.warps]
 
 
+def of_multiscale(dims, I1, I2, V, params):
+    """L1Huber_OF_MultiScale3D on the device; returns (V, iterations[levels, warps], coarsest level first)."""
+    dims = np.asarray(dims, np.int32)
+    n = int(np.prod(dims))
+    I1 = np.ascontiguousarray(I1, np.float32)
+    I2 = np.ascontiguousarray(I2, np.float32)
+    V = np.ascontiguousarray(V, np.float32).copy()
+    if I1.size != n or I2.size != n or V.size != 3 * n:
+        raise ValueError("I1, I2 need n voxels, V 3n")
+    its = np.zeros(max(params.levels * params.warps, 1), np.int32)
+    check(lib.stt_of_multiscale(dims.ctypes.data_as(_I), fptr(I1), fptr(I2), fptr(V), C.byref(params),
+                                its.ctypes.data_as(_I)))
+    return V, its[:params.levels * params.warps].reshape(params.levels, params.warps)
+
+
+def blur(dims, img, sigma, spectrum=1):
+    dims = np.asarray(dims, np.int32)
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(img)
+    check(lib.stt_blur(dims.ctypes.data_as(_I), int(spectrum), fptr(img), float(sigma), fptr(out)))
+    return out
+
+
+def resize(dims, img, new_dims, spectrum=1):
+    dims = np.asarray(dims, np.int32)
+    nd = np.asarray(new_dims, np.int32)
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros(int(np.prod(nd)) * spectrum, np.float32)
+    check(lib.stt_resize(dims.ctypes.data_as(_I), int(spectrum), fptr(img), nd.ctypes.data_as(_I), fptr(out)))
+    return out
+
+
 class Context:
     """Thin owner of an stt_ctx (one HIP stream, all solver state in HBM)."""
 
@@ -288,7 +324,7 @@ class Context:
         check(lib.stt_set_projections(self.h, frame, fptr(np.ascontiguousarray(b, np.float32).reshape(-1))))
 
     def get_buffer(self, which, frame):
-        n = 4 * self.V if which == STT_BUF_YTV else self.V
+        n = 4 * self.V if which == STT_BUF_YTV else (3 * self.V if which == STT_BUF_FLOW else self.V)
         out = np.empty(n, np.float32)
         check(lib.stt_get_buffer(self.h, which, frame, fptr(out)))
         return out
@@ -330,6 +366,12 @@ class Context:
         check(lib.stt_warp(self.h, fptr(np.ascontiguousarray(uvw, np.float32)),
                            fptr(np.ascontiguousarray(src, np.float32)), dt, fptr(out), interp))
         return out
+
+    def optical_flow(self, pair, params):
+        """L1Huber_OF_MultiScale3D between volumes pair and pair + 1 into flow pair (in place)."""
+        its = np.zeros(max(params.levels * params.warps, 1), np.int32)
+        check(lib.stt_optical_flow(self.h, pair, C.byref(params), its.ctypes.data_as(_I)))
+        return its[:params.levels * params.warps].reshape(params.levels, params.warps)
 
     def warp_vjp(self, uvw, src, dt, g, gsrc=None, guvw=None):
         """Adjoint of the trilinear warp: returns (gsrc, guvw) with the VJP

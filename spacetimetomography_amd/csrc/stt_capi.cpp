@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -141,6 +142,215 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
         if (residual) residual[w] = err;
     }
     HIPCHK(launch_of_roi_copy(Y, y, 10, g, roi, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return STT_OK;
+}
+
+// ---- the flow pyramid's image operators (CImg 1.8.0) on device buffers ----
+// Order-0 Deriche coefficients (CImg.h:32946-32968), fp32 with std::exp as the
+// reference computes them on the host.
+DericheCoef deriche_coef(float sigma) {
+    const float nnsigma = sigma < 0.1f ? 0.1f : sigma, alpha = 1.695f / nnsigma, ema = (float)std::exp(-alpha),
+                ema2 = (float)std::exp(-2 * alpha), b1 = -2 * ema, b2 = ema2;
+    const float k = (1 - ema) * (1 - ema) / (1 + 2 * alpha * ema - ema2);
+    DericheCoef c;
+    c.a0 = k;
+    c.a1 = k * (alpha - 1) * ema;
+    c.a2 = k * (alpha + 1) * ema;
+    c.a3 = -k * ema2;
+    c.b1 = b1;
+    c.b2 = b2;
+    c.coefp = (c.a0 + c.a1) / (1 + b1 + b2);
+    c.coefn = (c.a2 + c.a3) / (1 + b1 + b2);
+    return c;
+}
+
+// get_blur(sigma): Deriche along x, y, z (CImg.h:33249-33284); tmp holds n floats.
+int blur_device(const float* in, float* out, const int* d, int spectrum, float sigma, float* tmp, hipStream_t s) {
+    const size_t n = (size_t)d[0] * d[1] * d[2];
+    for (int ch = 0; ch < spectrum; ++ch) {
+        const float* src = in + ch * n;
+        float* dst = out + ch * n;
+        if (sigma < 0.1f) {
+            HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            continue;
+        }
+        const DericheCoef c = deriche_coef(sigma);
+        HIPCHK(launch_deriche(src, dst, d, 0, c, s));
+        HIPCHK(launch_deriche(dst, tmp, d, 1, c, s));
+        HIPCHK(launch_deriche(tmp, dst, d, 2, c, s));
+    }
+    return STT_OK;
+}
+
+// Tables of one axis of get_resize(..., 5) from w to ns samples: the moving
+// average's (source, weight) runs per output (CImg.h:28139-28150) or the
+// cubic's source index and fraction (CImg's double recurrence, :28448-28454).
+struct ResizeTables {
+    std::vector<int> seg0, seg_s, seg_d, sidx;
+    std::vector<double> frac;
+};
+ResizeTables resize_tables(int w, int ns) {
+    ResizeTables t;
+    if (w > ns) {
+        t.seg0.assign(ns + 1, 0);
+        unsigned a = (unsigned)w * ns, b = w, c = ns, src = 0, out = 0;
+        while (a) {
+            const unsigned dd = std::min(b, c);
+            a -= dd;
+            b -= dd;
+            c -= dd;
+            t.seg_s.push_back((int)src);
+            t.seg_d.push_back((int)dd);
+            if (!b) {
+                ++out;
+                t.seg0[out] = (int)t.seg_s.size();
+                b = w;
+            }
+            if (!c) {
+                ++src;
+                c = ns;
+            }
+        }
+    } else {
+        const double f = ns > 1 ? (w - 1.0) / (ns - 1) : 0;
+        double curr = 0;
+        for (int x = 0; x < ns; ++x) {
+            t.sidx.push_back((int)(unsigned)curr);
+            t.frac.push_back(curr - (unsigned)curr);
+            curr = std::min(w - 1.0, curr + f);
+        }
+    }
+    return t;
+}
+
+// get_resize(nd, spectrum, 5) of a d-sized image with `spectrum` planes;
+// axes x, y, z in turn; scratch holds 2 * max(n(d), n(nd)) floats.
+int resize_device(const float* in, const int* d, int spectrum, const int* nd, float* out, float* scratch,
+                  hipStream_t s) {
+    const size_t n = (size_t)d[0] * d[1] * d[2], nn = (size_t)nd[0] * nd[1] * nd[2];
+    const size_t half = std::max(n, nn);
+    ResizeTables tab[3];
+    size_t ints = 0, dbls = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (nd[a] == d[a]) continue;
+        tab[a] = resize_tables(d[a], nd[a]);
+        ints += tab[a].seg0.size() + tab[a].seg_s.size() + tab[a].seg_d.size() + tab[a].sidx.size();
+        dbls += tab[a].frac.size();
+    }
+    DevMem tm;
+    if (ints + dbls) HIPCHK(hipMalloc(&tm.p, dbls * sizeof(double) + ints * sizeof(int) + 64));
+    double* fbase = (double*)tm.p;
+    int* ibase = (int*)(fbase + dbls);
+    const int* tp[3][4] = {};
+    const double* fp[3] = {};
+    {
+        double* fcur = fbase;
+        int* icur = ibase;
+        for (int a = 0; a < 3; ++a) {
+            if (nd[a] == d[a]) continue;
+            const std::vector<int>* v[4] = {&tab[a].seg0, &tab[a].seg_s, &tab[a].seg_d, &tab[a].sidx};
+            for (int k = 0; k < 4; ++k) {
+                tp[a][k] = icur;
+                if (!v[k]->empty()) HIPCHK(hipMemcpy(icur, v[k]->data(), v[k]->size() * sizeof(int), hipMemcpyHostToDevice));
+                icur += v[k]->size();
+            }
+            fp[a] = fcur;
+            if (!tab[a].frac.empty())
+                HIPCHK(hipMemcpy(fcur, tab[a].frac.data(), tab[a].frac.size() * sizeof(double), hipMemcpyHostToDevice));
+            fcur += tab[a].frac.size();
+        }
+    }
+    int last = -1;
+    for (int a = 0; a < 3; ++a)
+        if (nd[a] != d[a]) last = a;
+    for (int ch = 0; ch < spectrum; ++ch) {
+        const float* cur = in + ch * n;
+        int cd[3] = {d[0], d[1], d[2]};
+        int ping = 0;
+        if (last < 0) {
+            HIPCHK(hipMemcpyAsync(out + ch * nn, cur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            continue;
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (nd[a] == cd[a]) continue;
+            float* dst = a == last ? out + ch * nn : scratch + ping * half;
+            HIPCHK(launch_resize_axis(cur, dst, cd, a, nd[a], tp[a][0], tp[a][1], tp[a][2], tp[a][3], fp[a], s));
+            cd[a] = nd[a];
+            cur = dst;
+            ping ^= 1;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));  // the tables are freed on return
+    return STT_OK;
+}
+
+// L1Huber_OF_MultiScale3D (optical_flow.h:630-734) on device buffers, with
+// Y1..Y4 = 0 as Render passes them (ST_Tomography.cpp:452-455).  V (3
+// planes, full size) in/out.  iters: levels x warps, coarsest level first.
+int of_multiscale_device(const int* D, const float* I1, const float* I2, float* V, const stt_of_params& p,
+                         hipStream_t s, int* iters) {
+    const size_t N = (size_t)D[0] * D[1] * D[2];
+    const int L = p.levels;
+    std::vector<std::array<int, 3>> dims(L);
+    std::vector<size_t> sz(L), off(L);
+    float t_eta = 1.0f;
+    size_t total = 0;
+    for (int k = 0; k < L; ++k) {  // finest first (:660-676)
+        for (int a = 0; a < 3; ++a) dims[k][a] = (int)((float)D[a] * t_eta);
+        for (int a = 0; a < 3; ++a)
+            if (dims[k][a] < 4) return fail(STT_ERR_INVALID, "a pyramid level is smaller than 4 voxels");
+        sz[k] = (size_t)dims[k][0] * dims[k][1] * dims[k][2];
+        off[k] = total;
+        total += 5 * sz[k];  // im1, im2, V (3)
+        t_eta = t_eta * p.eta;
+    }
+    DevMem mem;
+    // levels | blurred I1, I2 | Y (10, finest size) | Y' (10) | scratch (2 N, >= every resize)
+    const size_t nf = total + 2 * N + 20 * N + 2 * N;
+    HIPCHK(hipMalloc(&mem.p, nf * sizeof(float)));
+    float* lv = (float*)mem.p;
+    float* b1 = lv + total;
+    float* b2 = b1 + N;
+    float* Y = b2 + N;
+    float* Yn = Y + 10 * N;
+    float* scratch = Yn + 10 * N;
+    int rc = blur_device(I1, b1, D, 1, p.sigma, scratch, s);
+    if (!rc) rc = blur_device(I2, b2, D, 1, p.sigma, scratch, s);
+    for (int k = 0; k < L && !rc; ++k) {
+        float* im1 = lv + off[k];
+        rc = resize_device(b1, D, 1, dims[k].data(), im1, scratch, s);
+        if (!rc) rc = resize_device(b2, D, 1, dims[k].data(), im1 + sz[k], scratch, s);
+        if (!rc) rc = resize_device(V, D, 3, dims[k].data(), im1 + 2 * sz[k], scratch, s);
+    }
+    if (rc) return rc;
+    const float ext[3] = {(float)D[0], (float)D[1], (float)D[2]};
+    for (int j = 0; j < L; ++j) {  // coarsest first (:690-729)
+        const int k = L - 1 - j;
+        const int* nd = dims[k].data();
+        const VolGeom g = [&] {
+            VolGeom v{};
+            v.nx = nd[0];
+            v.ny = nd[1];
+            v.nz = nd[2];
+            return v;
+        }();
+        if (j == 0) HIPCHK(hipMemsetAsync(Y, 0, 10 * sz[k] * sizeof(float), s));
+        int roi[6];
+        for (int a = 0; a < 6; ++a) roi[a] = (int)((float)p.cut[a] * ((float)nd[a / 2] / (float)D[a / 2]));  // :700-705
+        float* im1 = lv + off[k];
+        rc = of_level_device(g, ext, im1, im1 + sz[k], im1 + 2 * sz[k], Y, p, roi, s,
+                             iters ? iters + j * p.warps : nullptr, nullptr);
+        if (rc) return rc;
+        if (k > 0) {  // :712-722
+            float* vnext = lv + off[k - 1] + 2 * sz[k - 1];
+            rc = resize_device(im1 + 2 * sz[k], nd, 3, dims[k - 1].data(), vnext, scratch, s);
+            if (!rc) rc = resize_device(Y, nd, 10, dims[k - 1].data(), Yn, scratch, s);
+            if (rc) return rc;
+            std::swap(Y, Yn);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(V, lv + off[0] + 2 * sz[0], 3 * N * sizeof(float), hipMemcpyDeviceToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     return STT_OK;
 }
@@ -701,6 +911,11 @@ static int buffer_of(stt_ctx* ctx, int which, int frame, float** ptr, size_t* n)
         case STT_BUF_YTV: *ptr = ctx->ytv[frame]; *n = 4 * ctx->V; break;
         case STT_BUF_YTP: *ptr = ctx->ytp[frame]; *n = ctx->V; break;
         case STT_BUF_YBC: *ptr = ctx->ybc[frame]; *n = ctx->V; break;
+        case STT_BUF_FLOW:
+            if (frame >= ctx->F) return fail(STT_ERR_INVALID, "flow pair out of range");
+            *ptr = ctx->flow[frame];
+            *n = 3 * ctx->V;
+            break;
         default: return fail(STT_ERR_INVALID, "unknown buffer id");
     }
     return STT_OK;
@@ -1268,7 +1483,7 @@ int stt_power_iteration(stt_ctx* ctx, int iters, double* L) {
 
 void stt_of_params_reference(stt_of_params* p) {
     if (!p) return;
-    // ST_Tomography.cpp:443-456; the ROI margins default to optical_flow.h:24
+    // ST_Tomography.cpp:443-456
     p->tol = 0.1f;
     p->lambda = 1.2f;
     p->huber = 1.0f;
@@ -1278,7 +1493,109 @@ void stt_of_params_reference(stt_of_params* p) {
     p->norm = 4;
     p->warps = 3;
     p->levels = 3;
-    for (int k = 0; k < 6; ++k) p->cut[k] = 1;
+    const int cut[6] = {8, 8, 8, 45, 8, 8};  // the global cut[] Render passes (ST_Tomography.cpp:114, :455)
+    for (int k = 0; k < 6; ++k) p->cut[k] = cut[k];
+}
+
+namespace {
+int check_dims(const int* d, int lo) {
+    for (int c = 0; c < 3; ++c)
+        if (d[c] < lo || d[c] > 4096) return fail(STT_ERR_INVALID, "each dimension must be in [" + std::to_string(lo) + ", 4096]");
+    if ((size_t)d[0] * d[1] * d[2] * 16 * sizeof(float) >= (size_t)1 << 32)
+        return fail(STT_ERR_INVALID, "a volume must stay below 2^26 voxels");
+    return STT_OK;
+}
+int check_device() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(STT_ERR_NODEVICE, "no HIP device");
+    return STT_OK;
+}
+int check_of_params(const stt_of_params* p) {
+    if (!p) return fail(STT_ERR_INVALID, "null flow parameters");
+    if (p->warps < 0 || p->max_iter < 0 || !(p->lambda > 0.0f) || p->levels < 1 || !(p->eta > 0.0f && p->eta <= 1.0f))
+        return fail(STT_ERR_INVALID, "bad flow parameters");
+    for (int k = 0; k < 6; ++k)
+        if (p->cut[k] < 0) return fail(STT_ERR_INVALID, "ROI margins must be >= 0");
+    return STT_OK;
+}
+}  // namespace
+
+int stt_blur(const int dims[3], int spectrum, const float* img, float sigma, float* out) {
+    if (!dims || !img || !out || spectrum < 1) return fail(STT_ERR_INVALID, "bad argument");
+    int rc = check_dims(dims, 1);
+    if (!rc) rc = check_device();
+    if (rc) return rc;
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    OwnedStream st;
+    HIPCHK(hipStreamCreate(&st.s));
+    DevMem buf;
+    HIPCHK(hipMalloc(&buf.p, (2 * (size_t)spectrum + 1) * n * sizeof(float)));
+    float* din = (float*)buf.p;
+    float* dout = din + spectrum * n;
+    HIPCHK(hipMemcpyAsync(din, img, spectrum * n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    rc = blur_device(din, dout, dims, spectrum, sigma, dout + spectrum * n, st.s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, spectrum * n * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    return STT_OK;
+}
+
+int stt_resize(const int dims[3], int spectrum, const float* img, const int new_dims[3], float* out) {
+    if (!dims || !new_dims || !img || !out || spectrum < 1) return fail(STT_ERR_INVALID, "bad argument");
+    int rc = check_dims(dims, 2);
+    if (!rc) rc = check_dims(new_dims, 2);
+    if (!rc) rc = check_device();
+    if (rc) return rc;
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2], nn = (size_t)new_dims[0] * new_dims[1] * new_dims[2];
+    OwnedStream st;
+    HIPCHK(hipStreamCreate(&st.s));
+    DevMem buf;
+    HIPCHK(hipMalloc(&buf.p, (spectrum * (n + nn) + 2 * std::max(n, nn)) * sizeof(float)));
+    float* din = (float*)buf.p;
+    float* dout = din + spectrum * n;
+    HIPCHK(hipMemcpyAsync(din, img, spectrum * n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    rc = resize_device(din, dims, spectrum, new_dims, dout, dout + spectrum * nn, st.s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, spectrum * nn * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    return STT_OK;
+}
+
+int stt_of_multiscale(const int dims[3], const float* I1, const float* I2, float* V, const stt_of_params* p,
+                      int* iters) {
+    if (!dims || !I1 || !I2 || !V) return fail(STT_ERR_INVALID, "null argument");
+    int rc = check_dims(dims, 4);
+    if (!rc) rc = check_of_params(p);
+    if (!rc) rc = check_device();
+    if (rc) return rc;
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    OwnedStream st;
+    HIPCHK(hipStreamCreate(&st.s));
+    DevMem buf;
+    HIPCHK(hipMalloc(&buf.p, 5 * n * sizeof(float)));
+    float* d1 = (float*)buf.p;
+    float* d2 = d1 + n;
+    float* dv = d2 + n;
+    HIPCHK(hipMemcpyAsync(d1, I1, n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    HIPCHK(hipMemcpyAsync(d2, I2, n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    HIPCHK(hipMemcpyAsync(dv, V, 3 * n * sizeof(float), hipMemcpyHostToDevice, st.s));
+    rc = of_multiscale_device(dims, d1, d2, dv, *p, st.s, iters);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(V, dv, 3 * n * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    return STT_OK;
+}
+
+int stt_optical_flow(stt_ctx* ctx, int pair, const stt_of_params* p, int* iters) {
+    CTX_CHECK(ctx);
+    if (pair < 0 || pair + 1 >= ctx->F) return fail(STT_ERR_INVALID, "frame pair out of range");
+    int rc = check_of_params(p);
+    if (rc) return rc;
+    const int dims[3] = {ctx->g.vol[0], ctx->g.vol[1], ctx->g.vol[2]};
+    rc = check_dims(dims, 4);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return of_multiscale_device(dims, ctx->x[pair], ctx->x[pair + 1], ctx->flow[pair], *p, ctx->stream, iters);
 }
 
 int stt_of_level(const int dims[3], const float ext[3], const float* I1, const float* I2, float* V, float* Y,

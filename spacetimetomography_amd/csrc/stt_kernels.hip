@@ -1973,6 +1973,83 @@ __global__ __launch_bounds__(256) void k_of_roi_copy(float* __restrict__ dst, co
     for (int m = 0; m < planes; ++m) dst[(size_t)m * n + q] = src[(size_t)m * n + q];
 }
 
+// ---- the flow pyramid's image operators (CImg 1.8.0) ----------------------
+// One line per thread of the order-0 Deriche filter with Neumann boundary
+// (_cimg_deriche_apply, CImg.h:32917-32989): forward pass into out, backward
+// pass adds its response.  Lines run along `axis` of a d0 x d1 x d2 volume.
+__global__ __launch_bounds__(256) void k_deriche(const float* __restrict__ in, float* __restrict__ out, int d0, int d1,
+                                                 int d2, int axis, DericheCoef c) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int dims[3] = {d0, d1, d2};
+    const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+    if (t >= dims[o1] * dims[o2]) return;
+    const int u = t % dims[o1], v = t / dims[o1];
+    const size_t st[3] = {1, (size_t)d0, (size_t)d0 * d1};
+    const size_t base = (size_t)u * st[o1] + (size_t)v * st[o2], off = st[axis];
+    const int N = dims[axis];
+    float xp = in[base], yb = c.coefp * xp, yp = yb;
+    for (int m = 0; m < N; ++m) {
+        const float xc = in[base + m * off];
+        const float yc = c.a0 * xc + c.a1 * xp - c.b1 * yp - c.b2 * yb;
+        out[base + m * off] = yc;
+        xp = xc;
+        yb = yp;
+        yp = yc;
+    }
+    float xn = in[base + (size_t)(N - 1) * off], xa = xn, yn = c.coefn * xn, ya = yn;
+    for (int n = N - 1; n >= 0; --n) {
+        const float xc = in[base + n * off];
+        const float yc = c.a2 * xn + c.a3 * xa - c.b1 * yn - c.b2 * ya;
+        xa = xn;
+        xn = xc;
+        ya = yn;
+        yn = yc;
+        out[base + n * off] = out[base + n * off] + yc;
+    }
+}
+
+// One axis of get_resize(..., 5): shrinking axes take the moving average
+// (CImg.h:28135-28150) whose output t sums in[seg_s[k]] * seg_d[k] over
+// k in [seg0[t], seg0[t+1]) and divides by the old length; growing axes take
+// the cubic of CImg.h:28441-28475 at source index sidx[t] + frac[t], in double.
+__global__ __launch_bounds__(256) void k_resize_axis(const float* __restrict__ in, float* __restrict__ out, int d0,
+                                                     int d1, int d2, int axis, int ns, const int* __restrict__ seg0,
+                                                     const int* __restrict__ seg_s, const int* __restrict__ seg_d,
+                                                     const int* __restrict__ sidx, const double* __restrict__ frac) {
+#pragma clang fp contract(off)
+    const int dims[3] = {d0, d1, d2};
+    int nd[3] = {d0, d1, d2};
+    nd[axis] = ns;
+    const size_t total = (size_t)nd[0] * nd[1] * nd[2];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int c[3];
+    c[0] = (int)(i % nd[0]);
+    c[1] = (int)((i / nd[0]) % nd[1]);
+    c[2] = (int)(i / ((size_t)nd[0] * nd[1]));
+    const int t = c[axis];
+    const size_t st[3] = {1, (size_t)d0, (size_t)d0 * d1};
+    c[axis] = 0;
+    const size_t base = (size_t)c[0] * st[0] + (size_t)c[1] * st[1] + (size_t)c[2] * st[2];
+    const size_t off = st[axis];
+    const int w = dims[axis];
+    if (w > ns) {
+        float acc = 0.0f;
+        for (int k = seg0[t]; k < seg0[t + 1]; ++k) acc += in[base + (size_t)seg_s[k] * off] * (float)(unsigned)seg_d[k];
+        out[i] = acc / (float)(unsigned)w;
+        return;
+    }
+    const long si = sidx[t];
+    const double tt = frac[t], val1 = in[base + si * off];
+    const double val0 = si > 0 ? (double)in[base + (si - 1) * off] : val1;
+    const double val2 = si <= w - 2 ? (double)in[base + (si + 1) * off] : val1;
+    const double val3 = si < w - 2 ? (double)in[base + (si + 2) * off] : val2;
+    const double val = val1 + (double)0.5f * (tt * (-val0 + val2) + tt * tt * (2 * val0 - 5 * val1 + 4 * val2 - val3) +
+                                              tt * tt * tt * (-val0 + 3 * val1 - 3 * val2 + val3));
+    out[i] = (float)val;
+}
+
 static inline dim3 vol_grid(const VolGeom& g, int nitems) {
     return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nitems);
 }
@@ -2152,6 +2229,25 @@ hipError_t launch_of_dual(bool res, const float* v, float* Y, float* Kx, const f
 hipError_t launch_of_roi_copy(float* dst, const float* src, int planes, const VolGeom& g, const int* roi,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_of_roi_copy, vol_grid(g, 1), dim3(256), 0, s, dst, src, planes, g, of_roi(roi));
+    return hipGetLastError();
+}
+
+
+hipError_t launch_deriche(const float* in, float* out, const int* dims, int axis, const DericheCoef& c, hipStream_t s) {
+    const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+    const int lines = dims[o1] * dims[o2];
+    hipLaunchKernelGGL(k_deriche, dim3((lines + 255) / 256), dim3(256), 0, s, in, out, dims[0], dims[1], dims[2], axis, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_resize_axis(const float* in, float* out, const int* dims, int axis, int ns, const int* seg0,
+                              const int* seg_s, const int* seg_d, const int* sidx, const double* frac,
+                              hipStream_t s) {
+    int nd[3] = {dims[0], dims[1], dims[2]};
+    nd[axis] = ns;
+    const size_t total = (size_t)nd[0] * nd[1] * nd[2];
+    hipLaunchKernelGGL(k_resize_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, dims[0],
+                       dims[1], dims[2], axis, ns, seg0, seg_s, seg_d, sidx, frac);
     return hipGetLastError();
 }
 

@@ -60,4 +60,15 @@ hipError_t launch_of_dual(bool res, const float* v, float* Y, float* Kx, const f
 hipError_t launch_of_roi_copy(float* dst, const float* src, int planes, const VolGeom& g, const int* roi,
                               hipStream_t s);
 
+// The flow pyramid's image operators: order-0 Deriche along one axis (one
+// thread per line) and one axis of CImg's interpolation-5 resize (tables from
+// the host: moving-average segments when shrinking, cubic taps when growing).
+struct DericheCoef {
+    float a0, a1, a2, a3, b1, b2, coefp, coefn;
+};
+hipError_t launch_deriche(const float* in, float* out, const int* dims, int axis, const DericheCoef& c, hipStream_t s);
+hipError_t launch_resize_axis(const float* in, float* out, const int* dims, int axis, int ns, const int* seg0,
+                              const int* seg_s, const int* seg_d, const int* sidx, const double* frac,
+                              hipStream_t s);
+
 }  // namespace stt

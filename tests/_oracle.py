@@ -61,6 +61,10 @@ _sigs = {
     "o_gradient_central": (C.c_int, [_I, _F, _F, _F, _F]),
     "o_of_level": (C.c_int, [_I, _F, _F, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_float,
                              _I, _I, _D]),
+    "o_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
+    "o_resize": (C.c_int, [_I, C.c_int, _F, _I, _F]),
+    "o_of_multiscale": (C.c_int, [_I, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_float,
+                                  C.c_float, C.c_int, C.c_float, _I, _I, _I]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(orc, _n)
@@ -285,3 +289,27 @@ def of_level(dims, ext, I1, I2, V, Y, tol=0.1, lam=1.2, max_iter=3000, norm=4, w
                    float(tol), float(lam), int(max_iter), int(norm), int(warps), float(huber),
                    i32(np.asarray(roi, np.int32)), i32(its), f64(errs))
     return V, Y, its, errs
+
+
+def blur(dims, img, sigma, spectrum=1):
+    out = np.zeros(int(np.prod(dims)) * spectrum, np.float32)
+    orc.o_blur(i32(np.asarray(dims, np.int32)), spectrum, f32(img), float(sigma), f32(out))
+    return out
+
+
+def resize(dims, img, new_dims, spectrum=1):
+    out = np.zeros(int(np.prod(new_dims)) * spectrum, np.float32)
+    orc.o_resize(i32(np.asarray(dims, np.int32)), spectrum, f32(img), i32(np.asarray(new_dims, np.int32)), f32(out))
+    return out
+
+
+def of_multiscale(dims, I1, I2, V, tol=0.1, lam=1.2, max_iter=3000, norm=4, warps=3, eta=0.65, sigma=0.5, levels=3,
+                  huber=1.0, cut=(8, 8, 8, 45, 8, 8)):
+    """L1Huber_OF_MultiScale3D; returns (V, level dims, iterations per level and warp)."""
+    V = np.ascontiguousarray(V, np.float32).copy()
+    ld = np.zeros(3 * levels, np.int32)
+    its = np.zeros(levels * warps, np.int32)
+    orc.o_of_multiscale(i32(np.asarray(dims, np.int32)), f32(I1), f32(I2), f32(V), float(tol), float(lam),
+                        int(max_iter), int(norm), int(warps), float(eta), float(sigma), int(levels), float(huber),
+                        i32(np.asarray(cut, np.int32)), i32(ld), i32(its))
+    return V, ld.reshape(levels, 3), its.reshape(levels, warps)

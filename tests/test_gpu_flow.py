@@ -65,3 +65,67 @@ def test_of_level_stops_at_the_residual_threshold():
     V, Y, its, res = stt.of_level(dims, dims, I1, I2, np.zeros(3 * n, np.float32), np.zeros(10 * n, np.float32), p)
     print(f"[of-stop] iterations {its.tolist()} residual {res.tolist()}", flush=True)
     assert its[0] % 50 == 0 and its[0] <= 3001 and res[0] <= 1e-3
+
+
+@pytest.mark.parametrize("spectrum", [1, 3])
+def test_blur_matches_oracle(spectrum):
+    dims = (21, 17, 19)
+    n = int(np.prod(dims))
+    img = np.random.default_rng(40 + spectrum).standard_normal(n * spectrum).astype(np.float32)
+    print(f"[blur] dims={dims} spectrum={spectrum}", flush=True)
+    assert np.array_equal(stt.blur(dims, img, 0.5, spectrum), O.blur(dims, img, 0.5, spectrum))
+
+
+@pytest.mark.parametrize("new", [(13, 11, 12), (34, 27, 30), (21, 9, 25), (166, 17, 19)])
+def test_resize_matches_oracle(new):
+    dims = (21, 17, 19)
+    n = int(np.prod(dims))
+    img = np.random.default_rng(50).standard_normal(3 * n).astype(np.float32)
+    print(f"[resize] {dims} -> {new}", flush=True)
+    for spectrum in (1, 3):
+        a = stt.resize(dims, img[:n * spectrum], new, spectrum)
+        b = O.resize(dims, img[:n * spectrum], new, spectrum)
+        assert np.array_equal(a, b), (new, spectrum, float(np.abs(a - b).max()))
+
+
+def _ms_params(max_iter, warps, cut):
+    p = stt.of_params_reference()
+    p.max_iter, p.warps, p.tol, p.lambda_ = max_iter, warps, 1e-9, 0.3
+    p.cut[:] = cut
+    return p
+
+
+def test_of_multiscale_matches_oracle():
+    dims = (26, 22, 20)
+    n = int(np.prod(dims))
+    I1, I2 = _pair(dims, (0.9, -0.6, 0.5), seed=77)
+    V0 = np.zeros(3 * n, np.float32)
+    cut = (2, 2, 2, 5, 2, 2)
+    p = _ms_params(60, 2, cut)
+    print(f"[of-multiscale] dims={dims} levels={p.levels} eta={p.eta} max_iter=60 warps=2 cut={cut}", flush=True)
+    V, its = stt.of_multiscale(dims, I1, I2, V0, p)
+    oV, ld, oits = O.of_multiscale(dims, I1, I2, V0, tol=p.tol, lam=p.lambda_, max_iter=60, warps=2, cut=cut)
+    print(f"  levels {ld.tolist()} iterations gpu {its.tolist()} oracle {oits.tolist()} relL2 V {rel_l2(V, oV):.2e}",
+          flush=True)
+    assert np.array_equal(its, oits)
+    assert rel_l2(V, oV) <= 1e-4
+
+
+def test_context_optical_flow_equals_host_form():
+    dims = (24, 20, 18)
+    n = int(np.prod(dims))
+    I1, I2 = _pair(dims, (0.5, 0.3, -0.4), seed=78)
+    g = stt.Geometry.make(dims, (16, 16))
+    ctx = stt.Context(g, stt.params_default(), [1, 1, 1])
+    ctx.set_volume(1, I1)
+    ctx.set_volume(2, I2)
+    V0 = (0.1 * np.random.default_rng(79).standard_normal(3 * n)).astype(np.float32)
+    ctx.set_flow(1, V0)
+    p = _ms_params(40, 1, (1,) * 6)
+    print(f"[ctx-flow] dims={dims} pair=1", flush=True)
+    its = ctx.optical_flow(1, p)
+    got = ctx.get_buffer(stt.STT_BUF_FLOW, 1)
+    ref, rits = stt.of_multiscale(dims, I1, I2, V0, p)
+    ctx.close()
+    assert np.array_equal(its, rits)
+    assert np.array_equal(got, ref)

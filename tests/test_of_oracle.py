@@ -101,3 +101,57 @@ def test_of_level_recovers_a_translation():
     for c in range(3):
         m = float(V[c * n:(c + 1) * n].reshape(dims[::-1])[inner].mean())
         assert abs(m - s[c]) < 0.12, (c, m, s[c])
+
+
+@needs_ref
+@pytest.mark.parametrize("spectrum", [1, 3])
+def test_blur_matches_cimg(spectrum):
+    dims = (13, 9, 11)
+    img = _vol(dims, 30)[:int(np.prod(dims))]
+    img = np.tile(img, spectrum) * np.repeat(np.arange(1, spectrum + 1, dtype=np.float32), int(np.prod(dims)))
+    ref = np.zeros_like(img)
+    O.ref.r_blur(O.i32(np.asarray(dims, np.int32)), spectrum, O.f32(img), 0.5, O.f32(ref))
+    assert np.array_equal(O.blur(dims, img, 0.5, spectrum), ref)
+
+
+@needs_ref
+@pytest.mark.parametrize("new", [(8, 6, 7), (20, 15, 18), (13, 5, 16), (166 // 7, 9, 11)])
+def test_resize_matches_cimg(new):
+    """get_resize(..., interpolation 5): moving average on shrinking axes, cubic on growing ones."""
+    dims = (13, 9, 11)
+    img = _vol(dims, 31)
+    for spectrum in (1, 3):
+        im = np.tile(img, spectrum)
+        ref = np.zeros(int(np.prod(new)) * spectrum, np.float32)
+        O.ref.r_resize(O.i32(np.asarray(dims, np.int32)), spectrum, O.f32(im), new[0], new[1], new[2], 5, O.f32(ref))
+        assert np.array_equal(O.resize(dims, im, new, spectrum), ref), (new, spectrum)
+
+
+def test_resize_moving_average_closed_form():
+    """Halving an axis averages pairs; resizing a constant keeps it."""
+    dims = (8, 4, 6)
+    img = _vol(dims, 32)
+    out = O.resize(dims, img, (4, 4, 6)).reshape(6, 4, 4)
+    a = img.reshape(6, 4, 8)
+    np.testing.assert_allclose(out, (a[:, :, 0::2] + a[:, :, 1::2]) / 2, rtol=1e-6)
+    const = np.full(int(np.prod(dims)), 2.5, np.float32)
+    for new in ((5, 3, 4), (11, 7, 9)):
+        np.testing.assert_allclose(O.resize(dims, const, new), 2.5, rtol=1e-6)
+
+
+def test_of_multiscale_level_sizes_and_translation():
+    """Pyramid sizes int(dims * eta^k), coarsest first; the flow stays in
+    full-resolution voxel units on every level (the warp keeps the finest
+    extent), so a translation of the whole image is recovered as is."""
+    dims = (24, 22, 20)
+    s = (0.8, -0.6, 0.5)
+    I1, I2 = _smooth_pair(dims, s)
+    n = int(np.prod(dims))
+    V, ld, its = O.of_multiscale(dims, I1, I2, np.zeros(3 * n, np.float32), lam=0.05, tol=1e-6, max_iter=150,
+                                 warps=2, cut=(1, 1, 1, 1, 1, 1))
+    assert ld.tolist() == [[10, 9, 8], [15, 14, 13], [24, 22, 20]]
+    inner = np.zeros(dims[::-1], bool)
+    inner[4:-4, 4:-4, 4:-4] = True
+    for c in range(3):
+        m = float(V[c * n:(c + 1) * n].reshape(dims[::-1])[inner].mean())
+        assert abs(m - s[c]) < 0.15, (c, m, s[c])
