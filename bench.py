@@ -156,6 +156,89 @@ def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
     }
 
 
+def flow_pair(stt, n_side, amp=2.0):
+    """Two synthetic frames of the deformed ellipsoid phantom (frames 0 and 1)."""
+    dims = (n_side,) * 3
+    I1 = stt.synth_phantom_frame(dims, 0, n_ellipsoids=10, seed=1, deform_seed=2, amp=amp)
+    I2 = stt.synth_phantom_frame(dims, 1, n_ellipsoids=10, seed=1, deform_seed=2, amp=amp)
+    return dims, I1, I2
+
+
+def flow_bench(args):
+    """Optical flow (L1Huber_OF) line: PD iterations/s of one 256^3 level from
+    two fixed-count runs (their difference cancels copies, warps and set-up),
+    the roofline of an iteration's primal + dual launches, the reference-
+    parameter pair solve time and the oracle's iteration rate."""
+    import spacetimetomography_amd as stt
+
+    n_side = 256
+    dims, I1, I2 = flow_pair(stt, n_side)
+    V = int(np.prod(dims))
+    p = stt.of_params_reference()
+    p.tol, p.warps = 0.0, 1  # no early stop: every run does exactly max_iter + 1 iterations
+    zeros3, zeros10 = np.zeros(3 * V, np.float32), np.zeros(10 * V, np.float32)
+    ext = np.asarray(dims, np.float32)
+
+    def timed(iters):
+        p.max_iter = iters
+        t = time.perf_counter()
+        stt.of_level(dims, ext, I1, I2, zeros3, zeros10, p, tuple(p.cut))
+        return time.perf_counter() - t
+
+    timed(49)  # warm-up (module load, graph instantiation)
+    lo, hi = 49 + 50 * args.warmup, 49 + 50 * (args.warmup + args.steps)
+    t_lo, t_hi = timed(lo), timed(hi)
+    per_iter = (t_hi - t_lo) / (hi - lo)
+    log(f"flow: {lo + 1} iterations {t_lo:.3f}s, {hi + 1} iterations {t_hi:.3f}s -> {per_iter * 1e3:.3f} ms/iteration")
+    # primal: reads y (10 planes), ux uy uz, tau (3), V (3), writes V (3) = 22 planes;
+    # dual: reads V (3), ux uy uz ut, sigma2, y (10), Kx (10), writes y (10), Kx (10) = 48 planes
+    it_bytes = 70 * 4 * V
+    achieved = it_bytes / per_iter
+    pr = stt.of_params_reference()
+    t = time.perf_counter()
+    _, its = stt.of_multiscale(dims, I1, I2, zeros3, pr)
+    t_pair = time.perf_counter() - t
+    log(f"flow: reference-parameter pair solve {t_pair:.2f}s, iterations per level/warp {its.tolist()}")
+    out = {
+        "metric": "optical flow PD iterations/s (L1/Huber-TV, one 256^3 level)",
+        "value": 1.0 / per_iter,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": hi - lo,
+        "warmup": lo,
+        "ms_per_step": per_iter * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (frames 0 and 1 of the seeded deformed ellipsoid phantom)",
+        "config": {"workload": "L1Huber_OF primal+dual iteration, 256x256x256 fp32, ROI cut 8,8,8,45,8,8"},
+        "roofline": {"bound": "hbm", "kernel": "k_of_primal + k_of_dual (one PD iteration)",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_iteration": it_bytes},
+        "pair_solve_s": t_pair,
+        "pair_iterations": its.tolist(),
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+
+        def otimed(iters):
+            t = time.perf_counter()
+            O.of_level(dims, ext, I1, I2, zeros3, zeros10, tol=0.0, lam=p.lambda_, max_iter=iters, warps=1,
+                       roi=tuple(p.cut))
+            return time.perf_counter() - t
+
+        a, b = otimed(4), otimed(14)
+        cpu_per_iter = (b - a) / 10
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": 1.0 / cpu_per_iter, "unit": "iters/s", "cores": threads, "kind": "port",
+                               "sample": "oracle L1Huber_OF at 256^3: 15 minus 5 iterations (difference)"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        log(f"flow cpu baseline {out['cpu_baseline']['value']:.3f} it/s")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,12 +254,18 @@ def main():
                     help="projector shape (0 slab, 64/32/16/8 level-synchronous); default: autotune")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (counter calibration)")
+    ap.add_argument("--flow", action="store_true",
+                    help="measure the optical flow solver instead (steps/warmup count 50-iteration chunks)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(frame-sharded multi-GPU is a design, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.flow:
+        flow_bench(args)
+        return
 
     import spacetimetomography_amd as stt
 
