@@ -315,6 +315,58 @@ int  stt_synth_flow(int nx, int ny, int nz, uint64_t deform_seed, float amp,
  * ((offset+k)*180*(3-sqrt 5)) mod 360, 2 = van der Corput base 2 scaled to 360. */
 int  stt_synth_angles(int kind, int n, int offset, float* out);
 
+/* ---- on-disk formats (host only, no device needed) ----------------------- */
+/* The reference reads and writes its data through CImg (built with libtiff):
+ * the projection stack and initial volumes with CImg(filename) -> load_tiff
+ * (ST_Tomography.cpp:355, VolumeData.cpp:119-122), reconstructions with
+ * save_tiff (:805), warped frames with save_tiff and flows with save_analyze
+ * (:464-468).  These functions replace those calls.  Images are returned in
+ * CImg's layout: img[x + W*(y + H*(z + D*c))], converted to fp32. */
+typedef struct stt_image_info {
+    int   width, height, depth, spectrum;  /* CImg dims x, y, z (TIFF pages), c */
+    int   bits;           /* bits per sample on disk (first page) */
+    int   sample_format;  /* 1 unsigned integer, 2 signed integer, 3 IEEE float */
+    float voxel[3];       /* voxel size: Analyze pixdim[1..3]; TIFF 1/resolution or 1 */
+} stt_image_info;
+/* Dimensions of the image CImg::load(path) would return (CImg.h:46206-46225):
+ * ".tif"/".tiff" -> TIFF, ".hdr"/".nii" (and ".img") -> Analyze 7.5/NIfTI-1,
+ * extension matched case-insensitively. */
+int  stt_image_query(const char* path, stt_image_info* info);
+/* Reads the image into out (n floats, n >= W*H*D*C).
+ * TIFF (CImg::load_tiff, CImg.h:47445-47765): every page becomes one z
+ * slice; samples convert to fp32 by CImg's type rule (8 bit: uint8 if
+ * SampleFormat = 1 else int8; 16 bit: uint16 / int16; 32 bit: uint32 /
+ * int32 / float; 64-bit float is accepted as well).  Strips or tiles,
+ * chunky or planar, either byte order, classic TIFF or BigTIFF; compression
+ * none, PackBits or LZW, predictor none or horizontal.  Pages of different
+ * sizes, palette/YCbCr/CMYK photometrics and bilevel images are rejected.
+ * Analyze (CImg::_load_analyze, CImg.h:47829-47955): datatypes 2, 4, 8, 16,
+ * 64 times the header's scale factor (0 read as 1), byte order from the
+ * header size; ".hdr" pairs with ".img", ".nii" reads at vox_offset. */
+int  stt_load_image(const char* path, float* out, size_t n);
+/* CImg::save_tiff of a float image (CImg.h:51554-51644): one page per z
+ * slice, c samples per pixel (chunky), 32-bit IEEE float, uncompressed,
+ * rows per strip = max(8192 / (W*c*4), 1) (libtiff's default strip size),
+ * tags Orientation 1, FillOrder 1, Photometric 1 (2 for c = 3 or 4),
+ * Software "CImg"; BigTIFF when the data reach 2 GiB. */
+int  stt_save_tiff(const char* path, const float* img, int w, int h, int d, int c);
+/* CImg::save_analyze of a float image (CImg.h:51715-51790): 348-byte header
+ * + ".img" data for ".hdr"/".img"/no extension (both names derived as CImg
+ * derives them), one 352-byte-header file for ".nii"; datatype 16, pixdim
+ * voxel[0..2] (NULL -> 1, 1, 1).  Dimensions must fit a short. */
+int  stt_save_analyze(const char* path, const float* img, int w, int h, int d, int c,
+                      const float* voxel);
+/* The start angles of ST_Tomography's constructor (ST_Tomography.cpp:258-267):
+ * whitespace-separated decimal numbers read frame-major into
+ * deg[f*nrounds + r].  A missing file, a malformed or non-finite number or
+ * fewer than nframes*nrounds values is an error; values after those are
+ * ignored. */
+int  stt_read_start_angles(const char* path, int nframes, int nrounds, float* deg);
+/* Block sizes of CImg::get_split(axis, nparts) with nparts > 0
+ * (CImg.h:31391-31420), used to cut the projection stack into frames
+ * (ST_Tomography.cpp:355): sizes[0..nparts-1], summing to size. */
+int  stt_split_sizes(int size, int nparts, int* sizes);
+
 #ifdef __cplusplus
 }
 #endif

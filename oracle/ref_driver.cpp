@@ -293,3 +293,65 @@ REF_API int r_resize(const int* dims, int spectrum, const float* img, int nx, in
     std::memcpy(out, r._data, r.size() * sizeof(float));
     return 0;
 }
+
+// ---- on-disk formats through CImg itself (CImg.h save_analyze / _load_analyze,
+// get_split).  TIFF needs libtiff headers, absent here, so it is not built.
+
+template <typename T>
+static int save_analyze_as(const char* path, const float* img, int w, int h, int d, int c, const float* voxel) {
+    CImg<T> im(w, h, d, c);
+    for (size_t i = 0; i < im.size(); ++i) im._data[i] = (T)img[i];
+    im.save_analyze(path, voxel);
+    return 0;
+}
+
+// datatype: 2 uchar, 4 short, 8 int, 16 float, 64 double (CImg picks it from T)
+REF_API int r_save_analyze(const char* path, const float* img, int w, int h, int d, int c, const float* voxel,
+                           int datatype) {
+    try {
+        switch (datatype) {
+            case 2: return save_analyze_as<unsigned char>(path, img, w, h, d, c, voxel);
+            case 4: return save_analyze_as<short>(path, img, w, h, d, c, voxel);
+            case 8: return save_analyze_as<int>(path, img, w, h, d, c, voxel);
+            case 16: return save_analyze_as<float>(path, img, w, h, d, c, voxel);
+            case 64: return save_analyze_as<double>(path, img, w, h, d, c, voxel);
+            default: return 1;
+        }
+    } catch (...) {
+        return 2;
+    }
+}
+
+// Loads with CImg<float>::load_analyze; dims[4] = w, h, d, c; voxel[3].
+// out may be NULL to query the dims only.
+REF_API int r_load_analyze(const char* path, int* dims, float* voxel, float* out, size_t n) {
+    try {
+        CImg<float> im;
+        float vs[3] = {0, 0, 0};
+        im.load_analyze(path, vs);
+        dims[0] = im.width();
+        dims[1] = im.height();
+        dims[2] = im.depth();
+        dims[3] = im.spectrum();
+        if (voxel) std::memcpy(voxel, vs, sizeof vs);
+        if (out) {
+            if (n < im.size()) return 3;
+            std::memcpy(out, im._data, im.size() * sizeof(float));
+        }
+        return 0;
+    } catch (...) {
+        return 2;
+    }
+}
+
+// Depths of CImg::get_split('z', nb) over an image of depth `size`.
+REF_API int r_split_sizes(int size, int nb, int* sizes) {
+    try {
+        CImg<float> im(1, 1, size, 1, 0.f);
+        CImgList<float> parts = im.get_split('z', nb);
+        for (unsigned int k = 0; k < parts.size(); ++k) sizes[k] = parts[k].depth();
+        return (int)parts.size();
+    } catch (...) {
+        return -1;
+    }
+}

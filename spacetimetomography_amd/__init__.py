@@ -7,6 +7,7 @@ from ._lib import (  # noqa: F401
     Context, Geometry, Params, Timing, SttError, check, lib, LIB_PATH,
     params_default, params_reference, synth_phantom, synth_phantom_frame, synth_flow, synth_angles,
     view_rounds, lds_sequence, OfParams, of_params_reference, of_level, of_multiscale, blur, resize,
+    ImageInfo, image_query, load_image, save_tiff, save_analyze, read_start_angles, split_sizes,
     STT_OK, STT_ERR_INVALID, STT_ERR_HIP, STT_ERR_NOMEM, STT_ERR_STATE, STT_ERR_NODEVICE,
     STT_BP_VOXEL, STT_BP_RAY, STT_PRIOR_ATV, STT_PRIOR_TV, STT_PRIOR_STV,
     STT_INTERP_CUBIC_CLAMPED, STT_INTERP_TRILINEAR,

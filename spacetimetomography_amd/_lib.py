@@ -61,6 +61,12 @@ class Timing(C.Structure):
                 ("aux_ms", C.c_double), ("aux_launches", C.c_int)]
 
 
+class ImageInfo(C.Structure):
+    """stt_image_info: CImg dims (x, y, z, c), on-disk sample layout, voxel size."""
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("depth", C.c_int), ("spectrum", C.c_int),
+                ("bits", C.c_int), ("sample_format", C.c_int), ("voxel", C.c_float * 3)]
+
+
 class OfParams(C.Structure):
     """stt_of_params (L1Huber_OF_MultiScale3D's arguments)."""
     _fields_ = [("tol", C.c_float), ("lambda_", C.c_float), ("huber", C.c_float), ("eta", C.c_float),
@@ -126,6 +132,12 @@ _SIGS = {
     "stt_optical_flow": (C.c_int, [_P, C.c_int, C.POINTER(OfParams), _I]),
     "stt_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
     "stt_resize": (C.c_int, [_I, C.c_int, _F, _I, _F]),
+    "stt_image_query": (C.c_int, [C.c_char_p, C.POINTER(ImageInfo)]),
+    "stt_load_image": (C.c_int, [C.c_char_p, _F, C.c_size_t]),
+    "stt_save_tiff": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "stt_save_analyze": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]),
+    "stt_read_start_angles": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _F]),
+    "stt_split_sizes": (C.c_int, [C.c_int, C.c_int, _I]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -270,6 +282,61 @@ def resize(dims, img, new_dims, spectrum=1):
     img = np.ascontiguousarray(img, np.float32)
     out = np.zeros(int(np.prod(nd)) * spectrum, np.float32)
     check(lib.stt_resize(dims.ctypes.data_as(_I), int(spectrum), fptr(img), nd.ctypes.data_as(_I), fptr(out)))
+    return out
+
+
+def _path(path):
+    return os.fsencode(path)
+
+
+def image_query(path):
+    """Dimensions and sample layout of the image CImg::load(path) would return."""
+    info = ImageInfo()
+    check(lib.stt_image_query(_path(path), C.byref(info)))
+    return info
+
+
+def load_image(path):
+    """CImg::load of a .tif/.tiff/.hdr/.img/.nii file as float32 (c, z, y, x)."""
+    info = image_query(path)
+    shape = (info.spectrum, info.depth, info.height, info.width)
+    out = np.empty(shape, np.float32)
+    check(lib.stt_load_image(_path(path), fptr(out), out.size))
+    return out
+
+
+def _cimg_dims(img):
+    """(w, h, d, c) of an array shaped (x), (y, x), (z, y, x) or (c, z, y, x)."""
+    img = np.ascontiguousarray(img, np.float32)
+    shape = (1,) * (4 - img.ndim) + img.shape
+    c, d, h, w = shape
+    return img, w, h, d, c
+
+
+def save_tiff(path, img):
+    """CImg::save_tiff of a float image: one page per z slice."""
+    img, w, h, d, c = _cimg_dims(img)
+    check(lib.stt_save_tiff(_path(path), fptr(img), w, h, d, c))
+
+
+def save_analyze(path, img, voxel=None):
+    """CImg::save_analyze of a float image (.hdr/.img pair or single .nii)."""
+    img, w, h, d, c = _cimg_dims(img)
+    vox = None if voxel is None else np.asarray(voxel, np.float32)
+    check(lib.stt_save_analyze(_path(path), fptr(img), w, h, d, c, fptr(vox)))
+
+
+def read_start_angles(path, nframes, nrounds):
+    """lds.txt start angles, frame-major: array (nframes, nrounds) in degrees."""
+    out = np.empty((nframes, nrounds), np.float32)
+    check(lib.stt_read_start_angles(_path(path), int(nframes), int(nrounds), fptr(out)))
+    return out
+
+
+def split_sizes(size, nparts):
+    """Block sizes of CImg::get_split(axis, nparts) over an axis of `size`."""
+    out = np.empty(int(nparts), np.int32)
+    check(lib.stt_split_sizes(int(size), int(nparts), iptr(out)))
     return out
 
 

@@ -29,7 +29,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wall"]
 
-HOST_SOURCES = ["stt_geometry.cpp", "stt_synth.cpp"]
+HOST_SOURCES = ["stt_geometry.cpp", "stt_synth.cpp", "stt_io.cpp"]
 HIP_SOURCES = ["stt_kernels.hip", "stt_capi.cpp"]
 
 

@@ -37,6 +37,12 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+int stt::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIPCHK(expr)                                                                                    \
     do {                                                                                                \
         hipError_t e_ = (expr);                                                                         \

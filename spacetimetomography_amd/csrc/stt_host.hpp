@@ -1,9 +1,14 @@
 // stt_host.hpp — host-side helpers of the library (not part of the C-ABI).
 #pragma once
+#include <string>
+
 #include "../../include/stt.h"
 #include "stt_device.hpp"
 
 namespace stt {
+
+// Records msg as this thread's stt_last_error() text and returns code.
+int set_error(int code, const std::string& msg);
 
 // Quaternion (x, y, z, w) of a rotation by theta degrees about +y, as the
 // reference's InitViews builds it.

@@ -86,6 +86,9 @@ if os.path.exists(REF_PATH):
         "r_gradient_central": (C.c_int, [_I, _F, _F, _F, _F]),
         "r_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
         "r_resize": (C.c_int, [_I, C.c_int, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]),
+        "r_save_analyze": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F, C.c_int]),
+        "r_load_analyze": (C.c_int, [C.c_char_p, _I, _F, _F, C.c_size_t]),
+        "r_split_sizes": (C.c_int, [C.c_int, C.c_int, _I]),
     }.items():
         _f = getattr(ref, _n)
         _f.restype = _r
