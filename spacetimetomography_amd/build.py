@@ -31,6 +31,9 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-W
 
 HOST_SOURCES = ["stt_geometry.cpp", "stt_synth.cpp", "stt_io.cpp"]
 HIP_SOURCES = ["stt_kernels.hip", "stt_capi.cpp"]
+# the command-line driver: host code over the C-ABI only
+CLI_SOURCE = "stt_cli.cpp"
+CLI = os.path.join(LIBDIR, "stt-tomography")
 
 
 def _run(cmd):
@@ -56,8 +59,12 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"])
     os.replace(tmp, LIB)
+    tmp = CLI + ".tmp"
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", tmp, os.path.join(CSRC, CLI_SOURCE),
+          "-L" + LIBDIR, "-lstt", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CLI)
     return LIB
 
 
