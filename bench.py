@@ -239,6 +239,179 @@ def flow_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def io_bench(args):
+    """On-disk formats line: TIFF read/write throughput of the projection
+    stack of the 256^3 x 8 workload (512x512 detector, 8 x 45 pages, fp32,
+    one file as the reference's -i input) and Analyze write/read of one
+    3 x 256^3 flow field, through the library's codec; the same stack
+    through libtiff (Pillow with its libtiff codec, the TIFF library CImg's
+    load_tiff/save_tiff call) is the CPU baseline.  Files go to $TMPDIR (page
+    cache), so the figures are codec throughput, not disk throughput."""
+    import shutil
+    import tempfile
+
+    import spacetimetomography_amd as stt
+
+    W, H, Z = 512, 512, 8 * 45
+    rng = np.random.default_rng(5)
+    stack = rng.standard_normal((Z, H, W), dtype=np.float32)
+    flow = rng.standard_normal((3, 256, 256, 256), dtype=np.float32)
+    reps = max(args.steps, 1)
+    d = tempfile.mkdtemp(prefix="stt_io_", dir=os.environ.get("TMPDIR", "/tmp"))
+
+    def best(fn):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return min(ts)
+
+    try:
+        tif, hdr = os.path.join(d, "proj.tif"), os.path.join(d, "flow.field.hdr")
+        t_tw = best(lambda: stt.save_tiff(tif, stack))
+        got = stt.load_image(tif)
+        assert np.array_equal(got[0], stack), "TIFF round trip differs"
+        del got
+        t_tr = best(lambda: stt.load_image(tif))
+        t_aw = best(lambda: stt.save_analyze(hdr, flow))
+        t_ar = best(lambda: stt.load_image(hdr))
+        nbytes, fbytes = stack.nbytes, flow.nbytes
+        log(f"io: TIFF write {nbytes / t_tw / 1e9:.2f} GB/s, read {nbytes / t_tr / 1e9:.2f} GB/s; "
+            f"Analyze write {fbytes / t_aw / 1e9:.2f} GB/s, read {fbytes / t_ar / 1e9:.2f} GB/s")
+        out = {
+            "metric": "projection-stack TIFF read throughput (codec, page cache)",
+            "value": nbytes / t_tr / 1e9,
+            "unit": "GB/s",
+            "n_gpus": 0,
+            "steps": reps,
+            "warmup": 1,
+            "ms_per_step": t_tr * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic (seeded normal values)",
+            "config": {"workload": f"TIFF stack {W}x{H}x{Z} fp32 ({nbytes / 2**20:.0f} MiB), "
+                                   "Analyze flow 3x256^3 fp32"},
+            "tiff_write_gbs": nbytes / t_tw / 1e9,
+            "analyze_write_gbs": fbytes / t_aw / 1e9,
+            "analyze_read_gbs": fbytes / t_ar / 1e9,
+            "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        }
+        if not args.no_cpu_baseline:
+            from PIL import Image, TiffImagePlugin
+
+            TiffImagePlugin.WRITE_LIBTIFF = True
+            TiffImagePlugin.READ_LIBTIFF = True
+            ptif = os.path.join(d, "pil.tif")
+
+            def pil_write():
+                ims = [Image.fromarray(stack[z]) for z in range(Z)]
+                ims[0].save(ptif, save_all=True, append_images=ims[1:], compression="raw")
+
+            def pil_read():
+                with Image.open(ptif) as im:
+                    for z in range(im.n_frames):
+                        im.seek(z)
+                        np.asarray(im)
+
+            t_pw, t_pr = best(pil_write), best(pil_read)
+            with Image.open(tif) as im:  # libtiff reads the file written here
+                im.seek(Z - 1)
+                assert np.array_equal(np.asarray(im), stack[Z - 1])
+            out["cpu_baseline"] = {"value": nbytes / t_pr / 1e9, "unit": "GB/s", "cores": 1, "kind": "library",
+                                   "sample": "the same stack read page by page by libtiff through Pillow "
+                                             f"(write {nbytes / t_pw / 1e9:.2f} GB/s)"}
+            log(f"io baseline: libtiff read {nbytes / t_pr / 1e9:.2f} GB/s, write {nbytes / t_pw / 1e9:.2f} GB/s")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    print(json.dumps(out), flush=True)
+
+
+def cli_bench(args):
+    """Command-line driver line: the reference's Render loop end to end
+    (stt-tomography, default profile) on 8 frames of 256^3 with 4 rounds of 10
+    views per frame on a 512x512 detector: inputs written as the reference
+    reads them (one TIFF projection stack, one TIFF volume per frame, an
+    lds.txt), then `steps` outer iterations (flow of the 7 frame pairs with
+    their dumps; from the second on, 2 PD passes with per-frame dumps).
+    Reports the wall time per outer iteration after the first and the
+    driver's own split into flow, PD and dumps."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    import spacetimetomography_amd as stt
+    from spacetimetomography_amd.build import CLI
+
+    dims, det, F, R = (256, 256, 256), (512, 512), 8, 4
+    outer = max(args.steps, 2)
+    d = tempfile.mkdtemp(prefix="stt_cli_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        start = stt.synth_angles(1, F * R).reshape(F, R)  # golden-angle start of every round
+        g = stt.Geometry.make(dims, det, sid=600.0, sdd=1000.0)
+        ctx = stt.Context(g, stt.params_default(), [10 * R] * F)
+        stack = []
+        for f in range(F):
+            truth = stt.synth_phantom_frame(dims, f, n_ellipsoids=10, seed=1, deform_seed=2, amp=4.0)
+            ctx.set_view_quaternions(f, stt.view_rounds(start[f]))
+            ctx.set_volume(f, truth)
+            for v in range(10 * R):
+                stack.append(ctx.forward_project(f, v)[0].reshape(det[1], det[0]))
+            stt.save_tiff(os.path.join(d, f"init-{f:03d}.tif"), (0.8 * truth).reshape(dims[::-1]))
+        ctx.close()
+        stt.save_tiff(os.path.join(d, "proj.tif"), np.stack(stack))
+        del stack
+        with open(os.path.join(d, "lds.txt"), "w") as fh:
+            fh.write("\n".join(" ".join(repr(float(a)) for a in row) for row in start))
+        cmd = [CLI, "-i", "proj.tif", "-f", "256", "-f", "256", "-f", "256", "-u", "512", "-u", "512",
+               "--oxyz", "0", "--sdg", "0", "-v", str(F), "-r", str(R), "-g", "600", "-j", "1000",
+               "-s", "0.2", "-t", "0.2", "-b", "2", "-p", "1", "--joint-iters", str(outer),
+               "--init", "init-%03d.tif", "-o", "out/rec", "--flow-dir", "flow"]
+        log("cli:", " ".join(cmd))
+        t = time.perf_counter()
+        proc = subprocess.run(cmd, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        wall = time.perf_counter() - t
+        if proc.returncode != 0:
+            log(proc.stdout[-4000:])
+            sys.exit(proc.returncode)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    rows = [tuple(float(x) for x in m.groups()) for m in
+            re.finditer(r"outer iteration (\d+): flow ([\d.]+) s, PD ([\d.]+) s, warp \+ dumps ([\d.]+) s",
+                        proc.stdout)]
+    load = float(re.search(r"inputs read and uploaded in ([\d.]+) s", proc.stdout).group(1))
+    per_outer = [r[1] + r[2] + r[3] for r in rows]
+    for line in proc.stdout.splitlines():
+        if line.startswith(("outer iteration", "inputs read", "done in")):
+            log(line)
+    sec = float(np.mean(per_outer))
+    out = {
+        "metric": "CLI Render seconds per outer iteration (flow of every pair + 2 PD passes + dumps)",
+        "value": sec,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": len(per_outer),
+        "warmup": 1,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded deformed ellipsoid phantom; projections are its forward projections)",
+        "config": {"workload": "stt-tomography, 256x256x256 x 8 frames, 40 views per frame (4 rounds of 10), "
+                               "512x512 detector, reference flow parameters"},
+        "flow_s": float(np.mean([r[1] for r in rows])),
+        "pd_s": float(np.mean([r[2] for r in rows])),
+        "dumps_s": float(np.mean([r[3] for r in rows])),
+        "inputs_load_s": load,
+        "process_wall_s": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +429,10 @@ def main():
                     help="run one device-to-device copy of MIB MiB first (counter calibration)")
     ap.add_argument("--flow", action="store_true",
                     help="measure the optical flow solver instead (steps/warmup count 50-iteration chunks)")
+    ap.add_argument("--io", action="store_true",
+                    help="measure the TIFF / Analyze codec instead (steps = repetitions, best kept)")
+    ap.add_argument("--cli", action="store_true",
+                    help="measure the command-line driver end to end instead (steps = outer iterations, >= 2)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -265,6 +442,12 @@ def main():
 
     if args.flow:
         flow_bench(args)
+        return
+    if args.io:
+        io_bench(args)
+        return
+    if args.cli:
+        cli_bench(args)
         return
 
     import spacetimetomography_amd as stt

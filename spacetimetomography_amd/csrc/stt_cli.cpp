@@ -407,6 +407,7 @@ int run(const Config& c, const Plan& pl) {
         ~Guard() { stt_ctx_destroy(c); }
     } guard{ctx};
 
+    double t_io = now_s();
     // SetData: projections split on z into frames (ST_Tomography.cpp:355)
     {
         std::vector<float> stack((size_t)pl.Z * D);
@@ -436,6 +437,7 @@ int run(const Config& c, const Plan& pl) {
     float gmax = 0.f;
     API(stt_ctx_get_gmax(ctx, &gmax));
     std::printf("G_Max %.9g\n", gmax);
+    std::printf("inputs read and uploaded in %.3f s\n", now_s() - t_io);
 
     if (c.flow && F > 1 && mkdirs(c.flow_dir) != 0) return die("cannot create " + c.flow_dir);
     if (mkdirs(dir_of(c.out)) != 0) return die("cannot create " + dir_of(c.out));
@@ -444,6 +446,7 @@ int run(const Config& c, const Plan& pl) {
     const int dims[3] = {c.g.vol[0], c.g.vol[1], c.g.vol[2]};
     std::vector<int> its((size_t)c.ofp.levels * c.ofp.warps);
     for (int outer = 0; outer < c.outer; ++outer) {
+        double t_flow = 0.0, t_pd = 0.0, t_dump = 0.0;
         if (c.flow && F > 1) {
             std::printf("==============================================\n"
                         "Flow-Reconstruction-subOptimization starts\n"
@@ -451,6 +454,8 @@ int run(const Config& c, const Plan& pl) {
             for (int f = 0; f + 1 < F; ++f) {
                 const double t0 = now_s();
                 API(stt_optical_flow(ctx, f, &c.ofp, its.data()));
+                const double t1 = now_s();
+                t_flow += t1 - t0;
                 API(stt_get_buffer(ctx, STT_BUF_FLOW, f, uvw.data()));
                 API(stt_get_volume(ctx, f + 1, i2.data()));
                 API(stt_warp(ctx, uvw.data(), i2.data(), -1.0f, warped.data(), STT_INTERP_CUBIC_CLAMPED));
@@ -458,12 +463,16 @@ int run(const Config& c, const Plan& pl) {
                                              dims[1], dims[2], f, outer);
                 API(stt_save_tiff((base + ".tiff").c_str(), warped.data(), dims[0], dims[1], dims[2], 1));
                 API(stt_save_analyze((base + ".field.hdr").c_str(), uvw.data(), dims[0], dims[1], dims[2], 3, nullptr));
-                std::printf("flow pair %d: %.3f s, iterations per level/warp:", f, now_s() - t0);
+                t_dump += now_s() - t1;
+                std::printf("flow pair %d: %.3f s, iterations per level/warp:", f, t1 - t0);
                 for (int k : its) std::printf(" %d", k);
                 std::printf("\n");
             }
         }
-        if (outer == 0) continue;
+        if (outer == 0) {
+            std::printf("outer iteration 0: flow %.3f s, warp + dumps %.3f s\n", t_flow, t_dump);
+            continue;
+        }
         std::printf("==============================================\n"
                     "Image-Reconstruction-subOptimization starts\n"
                     "==============================================\n");
@@ -482,7 +491,9 @@ int run(const Config& c, const Plan& pl) {
                 }
             }
             API(stt_synchronize(ctx));
-            std::printf("PD pass %d of outer iteration %d: %.3f s\n", pass, outer, now_s() - t0);
+            const double t1 = now_s();
+            t_pd += t1 - t0;
+            std::printf("PD pass %d of outer iteration %d: %.3f s\n", pass, outer, t1 - t0);
             // ST_Tomography::SART's dump of each frame after its sweep (:786-805)
             for (int f = 0; f < F; ++f) {
                 API(stt_get_volume(ctx, f, vol.data()));
@@ -491,7 +502,9 @@ int run(const Config& c, const Plan& pl) {
                 std::printf("%s\n", name.c_str());
                 ++idx_sart[f];
             }
+            t_dump += now_s() - t1;
         }
+        std::printf("outer iteration %d: flow %.3f s, PD %.3f s, warp + dumps %.3f s\n", outer, t_flow, t_pd, t_dump);
     }
     return 0;
 }

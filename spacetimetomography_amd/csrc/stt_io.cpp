@@ -500,6 +500,15 @@ int decode_page(const Tiff& t, const Page& p, uint64_t z, uint64_t D, float* out
                 const uint64_t need = rows * row_bytes;
                 const uint64_t off = p.offs[idx], cnt = p.cnts[idx];
                 if (off > t.size || cnt > t.size - off) return t.err("strip or tile past the end of the file");
+                // native-order fp32 strips of one sample per pixel are rows of
+                // the output already: read them in place
+                if (!p.tiled && per_px == 1 && st == SType::F32 && p.comp == 1 && !t.swap && p.pred == 1) {
+                    const uint64_t c = p.planar == 2 ? s : 0;
+                    if (c >= C) continue;
+                    if (cnt < need) return t.err("short strip or tile");
+                    if (!read_at(t.fd, off, out + W * (y0 + H * (z + D * c)), (size_t)need)) return t.err("read error");
+                    continue;
+                }
                 buf.resize((size_t)need);
                 if (p.comp == 1) {
                     if (cnt < need) return t.err("short strip or tile");
@@ -836,6 +845,12 @@ int analyze_load(const char* path, float* out, size_t n) {
     const uint64_t bytes = total * (uint64_t)width;
     if (ah.vox_offset > size || bytes > size - ah.vox_offset)
         return set_error(STT_ERR_INVALID, std::string("Analyze '") + path + "': data shorter than the header says");
+    // x * 1.0f is x (a signalling NaN aside, which CImg would return quieted): read in place
+    if (ah.datatype == 16 && !ah.swap && ah.scale == 1.f) {
+        if (!read_at(f.fd, ah.vox_offset, out, (size_t)bytes))
+            return set_error(STT_ERR_INVALID, std::string("Analyze '") + path + "': read error");
+        return STT_OK;
+    }
     std::vector<uint8_t> raw;
     const uint64_t chunk = 1ull << 24;  // elements per read
     for (uint64_t i0 = 0; i0 < total; i0 += chunk) {
