@@ -190,10 +190,21 @@ def flow_bench(args):
     t_lo, t_hi = timed(lo), timed(hi)
     per_iter = (t_hi - t_lo) / (hi - lo)
     log(f"flow: {lo + 1} iterations {t_lo:.3f}s, {hi + 1} iterations {t_hi:.3f}s -> {per_iter * 1e3:.3f} ms/iteration")
-    # primal: reads y (10 planes), ux uy uz, tau (3), V (3), writes V (3) = 22 planes;
-    # dual: reads V (3), ux uy uz ut, sigma2, y (10), Kx (10), writes y (10), Kx (10) = 48 planes
-    it_bytes = 70 * 4 * V
+    # primal, on the ROI: reads y (10 planes), ux uy uz, tau (3), V (3), writes V' (3) = 22 planes;
+    # dual, on every voxel: reads V' (3), V (3), ux uy uz ut, sigma2, y (10), writes y (10) = 31 planes
+    c = p.cut
+    roi_vox = (n_side - c[0] - c[1]) * (n_side - c[2] - c[3]) * (n_side - c[4] - c[5])
+    it_bytes = 4 * (22 * roi_vox + 31 * V)
     achieved = it_bytes / per_iter
+    # measured HBM bytes of one primal + dual pair at this size (committed counter summary)
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic_flow256.json")) as f:
+            t = json.load(f)
+        if t.get("voxels") == V:
+            traffic = t["bytes_per_iteration"]
+    except (OSError, ValueError, KeyError):
+        pass
     pr = stt.of_params_reference()
     t = time.perf_counter()
     _, its = stt.of_multiscale(dims, I1, I2, zeros3, pr)
@@ -215,7 +226,7 @@ def flow_bench(args):
         "config": {"workload": "L1Huber_OF primal+dual iteration, 256x256x256 fp32, ROI cut 8,8,8,45,8,8"},
         "roofline": {"bound": "hbm", "kernel": "k_of_primal + k_of_dual (one PD iteration)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_iteration": it_bytes},
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_iteration": it_bytes},
         "pair_solve_s": t_pair,
         "pair_iterations": its.tolist(),
     }
@@ -440,6 +451,8 @@ def main():
               "(frame-sharded multi-GPU is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     if args.flow:
         flow_bench(args)
         return
@@ -453,8 +466,6 @@ def main():
     import spacetimetomography_amd as stt
 
     cfg = CONFIGS[args.config]
-    if args.calibrate_copy:
-        copy_calibration(args.calibrate_copy)
     t0 = time.time()
     g, p, ctx, angles, projections = setup(stt, cfg)
     log(f"setup done in {time.time() - t0:.1f}s")

@@ -85,58 +85,77 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nb = of_partials(g);
     DevMem mem;
-    // W | G(4) | T(4) | Kx(10) | Kty(3) | y(10) floats, then 2 nb + 2 doubles
-    const size_t nf = 32 * n;
+    // W | G(4) | T(4) | V'(3) | K4(1) | Kty(3) | y(10) floats, then 2 nb + 2 doubles
+    const size_t nf = 26 * n;
     HIPCHK(hipMalloc(&mem.p, nf * sizeof(float) + (2 * (size_t)nb + 2) * sizeof(double)));
     float* W = (float*)mem.p;
     float* G = W + n;
     float* T = G + 4 * n;
-    float* Kx = T + 4 * n;
-    float* Kty = Kx + 10 * n;
+    float* V2 = T + 4 * n;
+    float* K4 = V2 + 3 * n;
+    float* Kty = K4 + n;
     float* y = Kty + 3 * n;
     double* part = (double*)(W + nf);
     double* sums = part + 2 * (size_t)nb;
     HIPCHK(hipMemsetAsync(G, 0, 4 * n * sizeof(float), s));
-    HIPCHK(hipMemsetAsync(Kx, 0, 13 * n * sizeof(float), s));
+    HIPCHK(hipMemsetAsync(Kty, 0, 3 * n * sizeof(float), s));
     HIPCHK(hipMemcpyAsync(y, Y, 10 * n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    // iteration k reads buf[(k-1) & 1] and writes buf[k & 1]; both start as the input flow
+    HIPCHK(hipMemcpyAsync(V2, V, 3 * n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    float* const buf[2] = {V, V2};
     const float third = 1.0f / 3.0f;
     const float sigma1 = std::min(third, std::min(third, third));  // :170
     const float hf = 1.0f / (1.0f + sigma1 * p.huber / p.lambda);  // :183
     const int huber = p.norm == 4;
-    auto iteration = [&](int it) -> hipError_t {
+    // first: 1 for a level's first iteration, 2 for a later warp's first iteration
+    auto iteration = [&](int it, int first) -> hipError_t {
         const int mode = it % 50 == 49 ? 1 : (it % 50 == 0 ? 2 : 0);
-        hipError_t e = launch_of_primal(mode, V, y, Kty, G, T, part, g, roi, s);
-        if (e == hipSuccess) e = launch_of_dual(mode == 2, V, y, Kx, G, T, part + nb, g, sigma1, hf, p.lambda, huber, s);
+        const float* vin = buf[(it - 1) & 1];
+        float* vout = buf[it & 1];
+        hipError_t e = launch_of_primal(mode, vin, vout, y, Kty, G, T, part, g, roi, s);
+        if (e == hipSuccess)
+            e = launch_of_dual(mode == 2, first, vout, vin, y, K4, G, T, part + nb, g, sigma1, hf, p.lambda, huber, s);
         if (e == hipSuccess && mode == 2) e = launch_sum_partials(part, nb, sums, s);
         if (e == hipSuccess && mode == 2) e = launch_sum_partials(part + nb, nb, sums + 1, s);
         return e;
     };
-    // 50 iterations (the residual's period) as one graph
-    OwnedGraphExec chunk;
-    {
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipSuccess;
-        for (int i = 1; i <= 50 && e == hipSuccess; ++i) e = iteration(i);
+    // 50 iterations (the residual's period) as one graph, one per kind of
+    // first iteration; 50 is even, so every chunk starts and ends on buf[0]
+    OwnedGraphExec chunk[3];
+    auto chunk_for = [&](int first) -> hipError_t {
+        if (chunk[first].e) return hipSuccess;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        for (int i = 1; i <= 50 && e == hipSuccess; ++i) e = iteration(i, i == 1 ? first : 0);
         hipGraph_t graph = nullptr;
         const hipError_t ce = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = ce;
-        if (e == hipSuccess) e = hipGraphInstantiate(&chunk.e, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphInstantiate(&chunk[first].e, graph, nullptr, nullptr, 0);
         if (graph) (void)hipGraphDestroy(graph);
-        HIPCHK(e);
-    }
+        return e;
+    };
     double host[2];
+    int cur = 0;  // buffer holding the current flow
     for (int w = 0; w < p.warps; ++w) {
+        if (cur) {  // the last warp stopped after an odd iteration count
+            HIPCHK(hipMemcpyAsync(V, V2, 3 * n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            cur = 0;
+        }
+        const int first = w == 0 ? 1 : 2;
+        if (w > 0) HIPCHK(launch_of_k4(V, G, K4, g, s));
         HIPCHK(launch_of_warp(V, I2, W, g, ext, s));
         HIPCHK(launch_of_prep(W, I1, V, G, T, g, roi, s));
         int it = 0;
         double err = 1.0;
         while (err > p.tol && it <= p.max_iter) {  // :290
             if (it % 50 == 0 && it + 50 <= p.max_iter + 1) {
-                HIPCHK(hipGraphLaunch(chunk.e, s));
+                const int f = it == 0 ? first : 0;
+                HIPCHK(chunk_for(f));
+                HIPCHK(hipGraphLaunch(chunk[f].e, s));
                 it += 50;
             } else {
                 ++it;
-                HIPCHK(iteration(it));
+                HIPCHK(iteration(it, it == 1 ? first : 0));
             }
             if (it % 50 == 0) {
                 HIPCHK(hipMemcpyAsync(host, sums, sizeof(host), hipMemcpyDeviceToHost, s));
@@ -144,9 +163,11 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
                 err = (host[1] * host[1] + host[0] * host[0]) / (double)n;  // :485
             }
         }
+        cur = it & 1;
         if (iters) iters[w] = it;
         if (residual) residual[w] = err;
     }
+    if (cur) HIPCHK(hipMemcpyAsync(V, V2, 3 * n * sizeof(float), hipMemcpyDeviceToDevice, s));
     HIPCHK(launch_of_roi_copy(Y, y, 10, g, roi, s));
     HIPCHK(hipStreamSynchronize(s));
     return STT_OK;

@@ -1858,13 +1858,20 @@ __global__ __launch_bounds__(256) void k_of_prep(const float* __restrict__ W, co
 
 // tools.cpp differences (forward: 0 on the last index; backward: a at the
 // first index, -a[n-2] at the last).
+// Both loads are unconditional (the neighbour index is clamped to the voxel
+// itself at the border) and the border cases are selects on the values, so a
+// wave issues every load of a stencil before it waits on any of them.
 __device__ __forceinline__ float of_dm(const float* a, size_t q, size_t st, int i, int n) {
-    if (i > 0 && i < n - 1) return a[q] - a[q - st];
-    if (i == 0) return a[q];
-    return -a[q - st];
+    const float c = a[q];
+    const float p = a[i > 0 ? q - st : q];
+    const float d = c - p;
+    return (i > 0 && i < n - 1) ? d : (i == 0 ? c : -p);
 }
 __device__ __forceinline__ float of_dp(const float* a, size_t q, size_t st, int i, int n) {
-    return i < n - 1 ? a[q + st] - a[q] : 0.0f;
+    const float c = a[q];
+    const float nb = a[i < n - 1 ? q + st : q];
+    const float d = nb - c;
+    return i < n - 1 ? d : 0.0f;
 }
 
 __device__ __forceinline__ void of_block_partial(double acc, double* partial) {
@@ -1875,11 +1882,14 @@ __device__ __forceinline__ void of_block_partial(double acc, double* partial) {
 
 // Primal step on the ROI (:300-340).  MODE 1 keeps K^T y for the next
 // iteration's residual; MODE 2 adds this iteration's primal residual.
+// The flow alternates between two buffers: iteration k reads vin (the
+// previous iterate) and writes vout on the ROI; outside the ROI both buffers
+// hold the level's input flow, which no step changes.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_of_primal(float* __restrict__ v, const float* __restrict__ Y,
-                                                   float* __restrict__ kty, const float* __restrict__ G,
-                                                   const float* __restrict__ T, double* __restrict__ partial,
-                                                   VolGeom g, OfRoi roi) {
+__global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin, float* __restrict__ vout,
+                                                   const float* __restrict__ Y, float* __restrict__ kty,
+                                                   const float* __restrict__ G, const float* __restrict__ T,
+                                                   double* __restrict__ partial, VolGeom g, OfRoi roi) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1896,9 +1906,9 @@ __global__ __launch_bounds__(256) void k_of_primal(float* __restrict__ v, const 
             const float kt = -1.0f * of_dm(yc, q, 1, x, g.nx) - 1.0f * of_dm(yc + n, q, sy, y, g.ny) -
                              1.0f * of_dm(yc + 2 * n, q, sz, z, g.nz) + G[c * n + q] * y4;
             const float tc = T[c * n + q];
-            const float vo = v[c * n + q];
+            const float vo = vin[c * n + q];
             const float vn = vo - tc * kt;
-            v[c * n + q] = vn;
+            vout[c * n + q] = vn;
             if (MODE == 1) kty[c * n + q] = kt;
             if (MODE == 2) r = r + fabsf((vo - vn) / tc - kty[c * n + q] + kt);
         }
@@ -1908,11 +1918,21 @@ __global__ __launch_bounds__(256) void k_of_primal(float* __restrict__ v, const 
 }
 
 // Dual step on every voxel (:343-480); RES adds the dual residual.
-template <bool RES>
-__global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, float* __restrict__ Y,
-                                                 float* __restrict__ Kx, const float* __restrict__ G,
-                                                 const float* __restrict__ T, double* __restrict__ partial,
-                                                 VolGeom g, float sigma1, float hf, float lambda, int huber) {
+// The reference keeps Kx = K v of the previous iteration in ten planes
+// (:379-422) and extrapolates with 2 K v_new - Kx_old.  K is linear, so the
+// old values are recomputed here from the previous iterate vold with the same
+// fp32 operations that produced them, which gives the same floats while
+// moving 3 planes instead of 20.  Two iterations differ:
+//   FIRST = 1  first iteration of a level: Kx_old = 0 (:229-242);
+//   FIRST = 2  first iteration of a later warp: the nine gradient terms are
+//              K vold as usual, but Kx4_old was formed with the previous
+//              warp's image gradient, kept in k4old (k_of_k4).
+template <bool RES, int FIRST>
+__global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, const float* __restrict__ vold,
+                                                 float* __restrict__ Y, const float* __restrict__ k4old,
+                                                 const float* __restrict__ G, const float* __restrict__ T,
+                                                 double* __restrict__ partial, VolGeom g, float sigma1, float hf,
+                                                 float lambda, int huber) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1921,44 +1941,74 @@ __global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, fl
     if (x < g.nx && y < g.ny) {
         const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
         const size_t q = (size_t)x + sy * y + sz * z;
-        float r = 0.0f;
+        // every load before the first store, so a wave has them all in flight
+        float kn[3][3], ko[3][3], yo[3][3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float* vc = v + (size_t)c * n;
-            const float kn[3] = {1.0f * of_dp(vc, q, 1, x, g.nx), 1.0f * of_dp(vc, q, sy, y, g.ny),
-                                 1.0f * of_dp(vc, q, sz, z, g.nz)};
-            float t[3], yo[3], ko[3];
+            const float* oc = vold + (size_t)c * n;
+            kn[c][0] = 1.0f * of_dp(vc, q, 1, x, g.nx);
+            kn[c][1] = 1.0f * of_dp(vc, q, sy, y, g.ny);
+            kn[c][2] = 1.0f * of_dp(vc, q, sz, z, g.nz);
+#pragma unroll
+            for (int e = 0; e < 3; ++e) ko[c][e] = 0.0f;
+            if (FIRST != 1) {
+                ko[c][0] = 1.0f * of_dp(oc, q, 1, x, g.nx);
+                ko[c][1] = 1.0f * of_dp(oc, q, sy, y, g.ny);
+                ko[c][2] = 1.0f * of_dp(oc, q, sz, z, g.nz);
+            }
+#pragma unroll
+            for (int e = 0; e < 3; ++e) yo[c][e] = Y[(size_t)(3 * c + e) * n + q];
+        }
+        const float g0 = G[q], g1 = G[n + q], g2 = G[2 * n + q], g3 = G[3 * n + q];
+        const float v0 = v[q], v1 = v[n + q], v2 = v[2 * n + q];
+        const float k4 = g0 * v0 + g1 * v1 + g2 * v2;
+        const float k4o = FIRST == 1 ? 0.0f
+                          : FIRST == 2 ? k4old[q]
+                                       : g0 * vold[q] + g1 * vold[n + q] + g2 * vold[2 * n + q];
+        const float s2 = T[3 * n + q];
+        const float y4o = Y[9 * n + q];
+        float r = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float t[3];
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
-                const size_t m = (size_t)(3 * c + e) * n + q;
-                yo[e] = Y[m];
-                ko[e] = Kx[m];
-                t[e] = yo[e] + sigma1 * (kn[e] + kn[e] - ko[e]);
+                t[e] = yo[c][e] + sigma1 * (kn[c][e] + kn[c][e] - ko[c][e]);
                 if (huber) t[e] = t[e] * hf;
             }
             const float div = fmaxf(1.0f, sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) / lambda);
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
-                const size_t m = (size_t)(3 * c + e) * n + q;
                 const float yn = t[e] / div;
-                Y[m] = yn;
-                Kx[m] = kn[e];
-                if (RES) r = r + fabsf((yo[e] - yn) / sigma1 - ko[e] + kn[e]);
+                Y[(size_t)(3 * c + e) * n + q] = yn;
+                if (RES) r = r + fabsf((yo[c][e] - yn) / sigma1 - ko[c][e] + kn[c][e]);
             }
         }
-        const float k4 = G[q] * v[q] + G[n + q] * v[n + q] + G[2 * n + q] * v[2 * n + q];
-        const float s2 = T[3 * n + q];
-        const float y4o = Y[9 * n + q], k4o = Kx[9 * n + q];
         // std::max(-1, std::min(1, a)): min(1, a) keeps 1 unless a < 1, so NaN gives 1
-        const float a = y4o + s2 * (k4 + k4 - k4o + G[3 * n + q]);
+        const float a = y4o + s2 * (k4 + k4 - k4o + g3);
         const float mn = a < 1.0f ? a : 1.0f;
         const float y4n = -1.0f < mn ? mn : -1.0f;
         Y[9 * n + q] = y4n;
-        Kx[9 * n + q] = k4;
         if (RES) r = r + fabsf((y4o - y4n) / s2 - k4o + k4);
         acc = (double)r;
     }
     if (RES) of_block_partial(acc, partial);
+}
+
+// Kx4 of the last iterate with the current image gradient, before the next
+// warp replaces the gradient (the value the reference's Kx4 plane carries
+// into that warp's first iteration).
+__global__ __launch_bounds__(256) void k_of_k4(const float* __restrict__ v, const float* __restrict__ G,
+                                               float* __restrict__ k4, VolGeom g) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+    k4[q] = G[q] * v[q] + G[n + q] * v[n + q] + G[2 * n + q] * v[2 * n + q];
 }
 
 // dst = src on the ROI, for `planes` planes (the write-back of :503-531).
@@ -2204,25 +2254,41 @@ hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float
     return hipGetLastError();
 }
 
-hipError_t launch_of_primal(int mode, float* v, const float* Y, float* kty, const float* G, const float* T,
-                            double* partial, const VolGeom& g, const int* roi, hipStream_t s) {
+hipError_t launch_of_primal(int mode, const float* vin, float* vout, const float* Y, float* kty, const float* G,
+                            const float* T, double* partial, const VolGeom& g, const int* roi, hipStream_t s) {
     const dim3 grid = vol_grid(g, 1);
+    const OfRoi r = of_roi(roi);
     switch (mode) {
-        case 1: hipLaunchKernelGGL(k_of_primal<1>, grid, dim3(256), 0, s, v, Y, kty, G, T, partial, g, of_roi(roi)); break;
-        case 2: hipLaunchKernelGGL(k_of_primal<2>, grid, dim3(256), 0, s, v, Y, kty, G, T, partial, g, of_roi(roi)); break;
-        default: hipLaunchKernelGGL(k_of_primal<0>, grid, dim3(256), 0, s, v, Y, kty, G, T, partial, g, of_roi(roi)); break;
+        case 1: hipLaunchKernelGGL(k_of_primal<1>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, T, partial, g, r); break;
+        case 2: hipLaunchKernelGGL(k_of_primal<2>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, T, partial, g, r); break;
+        default: hipLaunchKernelGGL(k_of_primal<0>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, T, partial, g, r); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_of_dual(bool res, const float* v, float* Y, float* Kx, const float* G, const float* T,
-                          double* partial, const VolGeom& g, float sigma1, float hf, float lambda, int huber,
-                          hipStream_t s) {
+hipError_t launch_of_dual(bool res, int first, const float* v, const float* vold, float* Y, const float* k4old,
+                          const float* G, const float* T, double* partial, const VolGeom& g, float sigma1, float hf,
+                          float lambda, int huber, hipStream_t s) {
     const dim3 grid = vol_grid(g, 1);
-    if (res)
-        hipLaunchKernelGGL(k_of_dual<true>, grid, dim3(256), 0, s, v, Y, Kx, G, T, partial, g, sigma1, hf, lambda, huber);
-    else
-        hipLaunchKernelGGL(k_of_dual<false>, grid, dim3(256), 0, s, v, Y, Kx, G, T, partial, g, sigma1, hf, lambda, huber);
+#define STT_OF_DUAL(R, F)                                                                                            \
+    hipLaunchKernelGGL((k_of_dual<R, F>), grid, dim3(256), 0, s, v, vold, Y, k4old, G, T, partial, g, sigma1, hf, \
+                       lambda, huber)
+    if (first == 1) {
+        if (res) STT_OF_DUAL(true, 1);
+        else STT_OF_DUAL(false, 1);
+    } else if (first == 2) {
+        if (res) STT_OF_DUAL(true, 2);
+        else STT_OF_DUAL(false, 2);
+    } else {
+        if (res) STT_OF_DUAL(true, 0);
+        else STT_OF_DUAL(false, 0);
+    }
+#undef STT_OF_DUAL
+    return hipGetLastError();
+}
+
+hipError_t launch_of_k4(const float* v, const float* G, float* k4, const VolGeom& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_of_k4, vol_grid(g, 1), dim3(256), 0, s, v, G, k4, g);
     return hipGetLastError();
 }
 

@@ -31,6 +31,8 @@ def _pair(dims, shift, seed):
     ((20, 18, 16), None, (1,) * 6, 120, 2),
     ((17, 22, 13), (26.0, 33.0, 20.0), (2, 1, 1, 3, 1, 2), 60, 2),
     ((24, 20, 18), None, (1,) * 6, 21, 1),
+    # 31 iterations per warp: odd counts and warp starts outside the 50-iteration graphs
+    ((14, 12, 10), None, (1, 2, 1, 1, 2, 1), 30, 3),
 ])
 def test_of_level_matches_oracle(case):
     dims, ext, roi, max_iter, warps = case
