@@ -190,11 +190,12 @@ def flow_bench(args):
     t_lo, t_hi = timed(lo), timed(hi)
     per_iter = (t_hi - t_lo) / (hi - lo)
     log(f"flow: {lo + 1} iterations {t_lo:.3f}s, {hi + 1} iterations {t_hi:.3f}s -> {per_iter * 1e3:.3f} ms/iteration")
-    # primal, on the ROI: reads y (10 planes), ux uy uz, tau (3), V (3), writes V' (3) = 22 planes;
-    # dual, on every voxel: reads V' (3), V (3), ux uy uz ut, sigma2, y (10), writes y (10) = 31 planes
+    # primal, on the ROI: reads y (10 planes), ux uy uz (3), V (3), writes V' (3) = 19 planes;
+    # dual, on every voxel: reads V' (3), V (3), ux uy uz ut (4), y (10), writes y (10) = 30 planes
+    # (the step sizes tau and sigma2 are recomputed from ux uy uz)
     c = p.cut
     roi_vox = (n_side - c[0] - c[1]) * (n_side - c[2] - c[3]) * (n_side - c[4] - c[5])
-    it_bytes = 4 * (22 * roi_vox + 31 * V)
+    it_bytes = 4 * (19 * roi_vox + 30 * V)
     achieved = it_bytes / per_iter
     # measured HBM bytes of one primal + dual pair at this size (committed counter summary)
     traffic = None

@@ -85,13 +85,12 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nb = of_partials(g);
     DevMem mem;
-    // W | G(4) | T(4) | V'(3) | K4(1) | Kty(3) | y(10) floats, then 2 nb + 2 doubles
-    const size_t nf = 26 * n;
+    // W | G(4) | V'(3) | K4(1) | Kty(3) | y(10) floats, then 2 nb + 2 doubles
+    const size_t nf = 22 * n;
     HIPCHK(hipMalloc(&mem.p, nf * sizeof(float) + (2 * (size_t)nb + 2) * sizeof(double)));
     float* W = (float*)mem.p;
     float* G = W + n;
-    float* T = G + 4 * n;
-    float* V2 = T + 4 * n;
+    float* V2 = G + 4 * n;
     float* K4 = V2 + 3 * n;
     float* Kty = K4 + n;
     float* y = Kty + 3 * n;
@@ -112,9 +111,9 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
         const int mode = it % 50 == 49 ? 1 : (it % 50 == 0 ? 2 : 0);
         const float* vin = buf[(it - 1) & 1];
         float* vout = buf[it & 1];
-        hipError_t e = launch_of_primal(mode, vin, vout, y, Kty, G, T, part, g, roi, s);
+        hipError_t e = launch_of_primal(mode, vin, vout, y, Kty, G, part, g, roi, s);
         if (e == hipSuccess)
-            e = launch_of_dual(mode == 2, first, vout, vin, y, K4, G, T, part + nb, g, sigma1, hf, p.lambda, huber, s);
+            e = launch_of_dual(mode == 2, first, vout, vin, y, K4, G, part + nb, g, sigma1, hf, p.lambda, huber, s);
         if (e == hipSuccess && mode == 2) e = launch_sum_partials(part, nb, sums, s);
         if (e == hipSuccess && mode == 2) e = launch_sum_partials(part + nb, nb, sums + 1, s);
         return e;
@@ -144,7 +143,7 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
         const int first = w == 0 ? 1 : 2;
         if (w > 0) HIPCHK(launch_of_k4(V, G, K4, g, s));
         HIPCHK(launch_of_warp(V, I2, W, g, ext, s));
-        HIPCHK(launch_of_prep(W, I1, V, G, T, g, roi, s));
+        HIPCHK(launch_of_prep(W, I1, V, G, g, roi, s));
         int it = 0;
         double err = 1.0;
         while (err > p.tol && it <= p.max_iter) {  // :290

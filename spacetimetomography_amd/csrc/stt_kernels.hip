@@ -1827,8 +1827,8 @@ __device__ __forceinline__ bool of_in_roi(const VolGeom& g, const OfRoi& r, int 
 // neighbours, CImg.h:34570-34580) and the linearised data term on the ROI
 // (:254-268); step sizes on every voxel (:273-287).
 __global__ __launch_bounds__(256) void k_of_prep(const float* __restrict__ W, const float* __restrict__ I1,
-                                                 const float* __restrict__ v, float* __restrict__ G,
-                                                 float* __restrict__ T, VolGeom g, OfRoi roi) {
+                                                 const float* __restrict__ v, float* __restrict__ G, VolGeom g,
+                                                 OfRoi roi) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1836,24 +1836,28 @@ __global__ __launch_bounds__(256) void k_of_prep(const float* __restrict__ W, co
     if (x >= g.nx || y >= g.ny) return;
     const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
     const size_t q = (size_t)x + sy * y + sz * z;
-    float ux = G[q], uy = G[n + q], uz = G[2 * n + q];
     if (of_in_roi(g, roi, x, y, z)) {
         const size_t xp = x + 1 < g.nx ? q + 1 : q, xm = x > 0 ? q - 1 : q;
         const size_t yp = y + 1 < g.ny ? q + sy : q, ym = y > 0 ? q - sy : q;
         const size_t zp = z + 1 < g.nz ? q + sz : q, zm = z > 0 ? q - sz : q;
-        ux = (W[xp] - W[xm]) / 2.0f;
-        uy = (W[yp] - W[ym]) / 2.0f;
-        uz = (W[zp] - W[zm]) / 2.0f;
+        const float ux = (W[xp] - W[xm]) / 2.0f;
+        const float uy = (W[yp] - W[ym]) / 2.0f;
+        const float uz = (W[zp] - W[zm]) / 2.0f;
         G[q] = ux;
         G[n + q] = uy;
         G[2 * n + q] = uz;
         G[3 * n + q] = W[q] - I1[q] - ux * v[q] - uy * v[n + q] - uz * v[2 * n + q];
     }
-    const float s = 4.0f / 1.0f;
-    T[q] = 1.0f / (s + fabsf(ux));
-    T[n + q] = 1.0f / (s + fabsf(uy));
-    T[2 * n + q] = 1.0f / (s + fabsf(uz));
-    T[3 * n + q] = 1.0f / (fabsf(ux) + fabsf(uy) + fabsf(uz));
+}
+
+// The per-voxel steps of :273-287 are functions of the gradient planes, which
+// the primal and the dual read anyway; they are recomputed there with the
+// reference's expressions instead of being stored (stepsize = 1).  Outside the
+// ROI the gradient keeps its zero fill, so tau = 1/4 and sigma2 = 1/0 = inf,
+// as the stored planes held.
+__device__ __forceinline__ float of_tau(float u) { return 1.0f / (4.0f / 1.0f + fabsf(u)); }
+__device__ __forceinline__ float of_sigma2(float ux, float uy, float uz) {
+    return 1.0f / (fabsf(ux) + fabsf(uy) + fabsf(uz));
 }
 
 // tools.cpp differences (forward: 0 on the last index; backward: a at the
@@ -1888,8 +1892,8 @@ __device__ __forceinline__ void of_block_partial(double acc, double* partial) {
 template <int MODE>
 __global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin, float* __restrict__ vout,
                                                    const float* __restrict__ Y, float* __restrict__ kty,
-                                                   const float* __restrict__ G, const float* __restrict__ T,
-                                                   double* __restrict__ partial, VolGeom g, OfRoi roi) {
+                                                   const float* __restrict__ G, double* __restrict__ partial,
+                                                   VolGeom g, OfRoi roi) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1903,9 +1907,10 @@ __global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float* yc = Y + (size_t)(3 * c) * n;
+            const float gc = G[c * n + q];
             const float kt = -1.0f * of_dm(yc, q, 1, x, g.nx) - 1.0f * of_dm(yc + n, q, sy, y, g.ny) -
-                             1.0f * of_dm(yc + 2 * n, q, sz, z, g.nz) + G[c * n + q] * y4;
-            const float tc = T[c * n + q];
+                             1.0f * of_dm(yc + 2 * n, q, sz, z, g.nz) + gc * y4;
+            const float tc = of_tau(gc);
             const float vo = vin[c * n + q];
             const float vn = vo - tc * kt;
             vout[c * n + q] = vn;
@@ -1930,9 +1935,8 @@ __global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin
 template <bool RES, int FIRST>
 __global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, const float* __restrict__ vold,
                                                  float* __restrict__ Y, const float* __restrict__ k4old,
-                                                 const float* __restrict__ G, const float* __restrict__ T,
-                                                 double* __restrict__ partial, VolGeom g, float sigma1, float hf,
-                                                 float lambda, int huber) {
+                                                 const float* __restrict__ G, double* __restrict__ partial, VolGeom g,
+                                                 float sigma1, float hf, float lambda, int huber) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1966,7 +1970,7 @@ __global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, co
         const float k4o = FIRST == 1 ? 0.0f
                           : FIRST == 2 ? k4old[q]
                                        : g0 * vold[q] + g1 * vold[n + q] + g2 * vold[2 * n + q];
-        const float s2 = T[3 * n + q];
+        const float s2 = of_sigma2(g0, g1, g2);
         const float y4o = Y[9 * n + q];
         float r = 0.0f;
 #pragma unroll
@@ -2248,30 +2252,30 @@ hipError_t launch_of_warp(const float* v, const float* img, float* out, const Vo
     return hipGetLastError();
 }
 
-hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, float* T, const VolGeom& g,
+hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, const VolGeom& g,
                           const int* roi, hipStream_t s) {
-    hipLaunchKernelGGL(k_of_prep, vol_grid(g, 1), dim3(256), 0, s, W, I1, v, G, T, g, of_roi(roi));
+    hipLaunchKernelGGL(k_of_prep, vol_grid(g, 1), dim3(256), 0, s, W, I1, v, G, g, of_roi(roi));
     return hipGetLastError();
 }
 
 hipError_t launch_of_primal(int mode, const float* vin, float* vout, const float* Y, float* kty, const float* G,
-                            const float* T, double* partial, const VolGeom& g, const int* roi, hipStream_t s) {
+                            double* partial, const VolGeom& g, const int* roi, hipStream_t s) {
     const dim3 grid = vol_grid(g, 1);
     const OfRoi r = of_roi(roi);
     switch (mode) {
-        case 1: hipLaunchKernelGGL(k_of_primal<1>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, T, partial, g, r); break;
-        case 2: hipLaunchKernelGGL(k_of_primal<2>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, T, partial, g, r); break;
-        default: hipLaunchKernelGGL(k_of_primal<0>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, T, partial, g, r); break;
+        case 1: hipLaunchKernelGGL(k_of_primal<1>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, partial, g, r); break;
+        case 2: hipLaunchKernelGGL(k_of_primal<2>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, partial, g, r); break;
+        default: hipLaunchKernelGGL(k_of_primal<0>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, partial, g, r); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_of_dual(bool res, int first, const float* v, const float* vold, float* Y, const float* k4old,
-                          const float* G, const float* T, double* partial, const VolGeom& g, float sigma1, float hf,
-                          float lambda, int huber, hipStream_t s) {
+                          const float* G, double* partial, const VolGeom& g, float sigma1, float hf, float lambda,
+                          int huber, hipStream_t s) {
     const dim3 grid = vol_grid(g, 1);
-#define STT_OF_DUAL(R, F)                                                                                            \
-    hipLaunchKernelGGL((k_of_dual<R, F>), grid, dim3(256), 0, s, v, vold, Y, k4old, G, T, partial, g, sigma1, hf, \
+#define STT_OF_DUAL(R, F)                                                                                          \
+    hipLaunchKernelGGL((k_of_dual<R, F>), grid, dim3(256), 0, s, v, vold, Y, k4old, G, partial, g, sigma1, hf, \
                        lambda, huber)
     if (first == 1) {
         if (res) STT_OF_DUAL(true, 1);

@@ -50,14 +50,14 @@ hipError_t launch_pi_apply_kt(const float* y, float* w, int F, const VolGeom& g,
 int of_partials(const VolGeom& g);
 hipError_t launch_of_warp(const float* v, const float* img, float* out, const VolGeom& g, const float* ext,
                           hipStream_t s);
-hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, float* T, const VolGeom& g,
+hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, const VolGeom& g,
                           const int* roi, hipStream_t s);
 hipError_t launch_of_primal(int mode, const float* vin, float* vout, const float* Y, float* kty, const float* G,
-                            const float* T, double* partial, const VolGeom& g, const int* roi, hipStream_t s);
+                            double* partial, const VolGeom& g, const int* roi, hipStream_t s);
 // first: 0 normal, 1 first iteration of a level, 2 first iteration of a later warp
 hipError_t launch_of_dual(bool res, int first, const float* v, const float* vold, float* Y, const float* k4old,
-                          const float* G, const float* T, double* partial, const VolGeom& g, float sigma1, float hf,
-                          float lambda, int huber, hipStream_t s);
+                          const float* G, double* partial, const VolGeom& g, float sigma1, float hf, float lambda,
+                          int huber, hipStream_t s);
 hipError_t launch_of_k4(const float* v, const float* G, float* k4, const VolGeom& g, hipStream_t s);
 hipError_t launch_of_roi_copy(float* dst, const float* src, int planes, const VolGeom& g, const int* roi,
                               hipStream_t s);
