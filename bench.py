@@ -86,6 +86,66 @@ def setup(stt, cfg):
     return g, p, ctx, angles, projections
 
 
+def group_bench(args):
+    """Configuration "uneven batched solves": 32 frames with seeded view
+    counts U{8..64} (seed 7) and volume sizes drawn from {128^3, 192^3}
+    (seed 8), 384x384 detector, van der Corput views; SART (one sweep) + PD
+    ATV, k = e = 0, so the frames are independent solves.  Frames of one size
+    share a context; both contexts run as one group whose iteration is one
+    hipGraph.  Reports group iterations per second (all 32 frames)."""
+    import spacetimetomography_amd as stt
+
+    F, det = 32, (384, 384)
+    P = [int(v) for v in np.random.default_rng(7).integers(8, 65, F)]
+    sizes = [int(v) for v in np.random.default_rng(8).choice([128, 192], F)]
+    ctxs, frames = [], {}
+    for n in sorted(set(sizes)):
+        idx = [f for f in range(F) if sizes[f] == n]
+        frames[n] = idx
+        vol = (n, n, n)
+        g = stt.Geometry.make(vol, det, (1.0, 1.0, 1.0), 1.0, 600.0, 1000.0)
+        p = stt.params_default()
+        p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.0, 0.0, 0.01, 0.3
+        p.prior = stt.STT_PRIOR_ATV
+        ctx = stt.Context(g, p, [P[f] for f in idx])
+        zero = np.zeros(n ** 3, np.float32)
+        for j, f in enumerate(idx):
+            ctx.set_views(j, stt.synth_angles(2, P[f], offset=sum(P[:f])))
+            ctx.set_volume(j, stt.synth_phantom_frame(vol, f, n_ellipsoids=10, seed=1, deform_seed=2, amp=3.0))
+            ctx.set_projections(j, np.stack([ctx.forward_project(j, k)[0] for k in range(P[f])]))
+        for j in range(len(idx) + 1):
+            ctx.set_volume(j, zero)
+        ctx.pd_init()
+        ctxs.append(ctx)
+        log(f"group: {len(idx)} frames of {n}^3, {sum(P[f] for f in idx)} views")
+    grp = stt.Group(ctxs)
+    grp.iterate(max(args.warmup, 1), use_graph=True)
+    ms = grp.iterate(args.steps, use_graph=True)
+    per = ms / args.steps
+    log(f"group: {args.steps} iterations {ms:.1f} ms -> {per:.2f} ms per iteration")
+    out = {
+        "metric": "uneven batched solves: group iterations/s (32 frames, mixed 128^3/192^3)",
+        "value": 1e3 / per,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": per,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantoms; projections are their forward projections)",
+        "config": {"workload": f"32 frames, views U{{8..64}} (total {sum(P)}), sizes 128^3 x "
+                               f"{len(frames.get(128, []))} and 192^3 x {len(frames.get(192, []))}, 384x384 "
+                               "detector, SART(1) + PD ATV, one hipGraph per group iteration"},
+    }
+    grp.close()
+    for c in ctxs:
+        c.close()
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB through the HIP runtime (a plain
     copy kernel of known size: reads and writes exactly mib MiB), so that a
@@ -445,6 +505,8 @@ def main():
                     help="measure the TIFF / Analyze codec instead (steps = repetitions, best kept)")
     ap.add_argument("--cli", action="store_true",
                     help="measure the command-line driver end to end instead (steps = outer iterations, >= 2)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="measure uneven batched solves of mixed 128^3/192^3 frames as one group instead")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -462,6 +524,9 @@ def main():
         return
     if args.cli:
         cli_bench(args)
+        return
+    if args.mixed:
+        group_bench(args)
         return
 
     import spacetimetomography_amd as stt

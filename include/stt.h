@@ -209,6 +209,21 @@ int  stt_pd_extrapolate(stt_ctx* ctx, int frame);
  * timing may be NULL; when given, kernels are bracketed with HIP events. */
 int  stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph,
                  stt_timing* timing);
+/* Groups of contexts.  Frames of different grids cannot share a context
+ * (the PD steps couple neighbouring frames of one grid), so independent
+ * solves of mixed volume sizes -- 128^3 and 192^3 frames with uneven view
+ * counts -- take one context per size.  A group runs their ST iterations as
+ * one unit: each context's batched iteration on its own streams, forked from
+ * and joined into the first context's stream, and with use_graph = 1 all of
+ * it captured in one hipGraph launched once per iteration.  Each context's
+ * results are those of stt_iterate(ctx, n, 1, ...).  No counterpart in the
+ * reference, which solves one grid.  The contexts must share a device and
+ * outlive the group; one group per thread.  total_ms (may be NULL) receives
+ * the time of the n_iters iterations (HIP events). */
+typedef struct stt_group stt_group;
+int  stt_group_create(stt_ctx* const* ctxs, int n, stt_group** out);
+int  stt_group_destroy(stt_group* grp);
+int  stt_group_iterate(stt_group* grp, int n_iters, int use_graph, double* total_ms);
 /* Projector shape (process-wide): 0 = slab projector (16x16 rays per block,
  * volume boxes staged in LDS; default); 64, 32, 16 or 8 = level-synchronous
  * projector with that many rays per detector row of a 64-lane wave. */

@@ -4,7 +4,7 @@ The native library is spacetimetomography_amd/lib/libstt.so; this package
 only binds it (ctypes).  See include/stt.h for the entry points.
 """
 from ._lib import (  # noqa: F401
-    Context, Geometry, Params, Timing, SttError, check, lib, LIB_PATH,
+    Context, Group, Geometry, Params, Timing, SttError, check, lib, LIB_PATH,
     params_default, params_reference, synth_phantom, synth_phantom_frame, synth_flow, synth_angles,
     view_rounds, lds_sequence, OfParams, of_params_reference, of_level, of_multiscale, blur, resize,
     ImageInfo, image_query, load_image, save_tiff, save_analyze, read_start_angles, split_sizes,

@@ -132,6 +132,9 @@ _SIGS = {
     "stt_optical_flow": (C.c_int, [_P, C.c_int, C.POINTER(OfParams), _I]),
     "stt_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
     "stt_resize": (C.c_int, [_I, C.c_int, _F, _I, _F]),
+    "stt_group_create": (C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(_P)]),
+    "stt_group_destroy": (C.c_int, [_P]),
+    "stt_group_iterate": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "stt_image_query": (C.c_int, [C.c_char_p, C.POINTER(ImageInfo)]),
     "stt_load_image": (C.c_int, [C.c_char_p, _F, C.c_size_t]),
     "stt_save_tiff": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -488,3 +491,35 @@ class Context:
         ms = (C.c_double * 5)()
         check(lib.stt_autotune(self.h, C.byref(best), ms))
         return best.value, list(ms)
+
+
+class Group:
+    """stt_group: several contexts (e.g. one per volume size) iterated as one
+    unit, one hipGraph for all of them with use_graph."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)  # keeps the contexts alive
+        arr = (_P * len(self.contexts))(*[c.h for c in self.contexts])
+        h = _P()
+        check(lib.stt_group_create(arr, len(self.contexts), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.stt_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def iterate(self, n=1, use_graph=True):
+        """n ST iterations of every context; returns their time in ms."""
+        ms = C.c_double()
+        check(lib.stt_group_iterate(self.h, int(n), int(use_graph), C.byref(ms)))
+        return ms.value

@@ -428,6 +428,7 @@ struct stt_ctx {
     // graph
     hipGraphExec_t graph_exec = nullptr;
     int graph_batched = -1;
+    uint64_t graph_gen = 0;  // bumped whenever captured launches would go stale (groups compare it)
     std::vector<void*> allocs;
 
     ~stt_ctx() {
@@ -704,6 +705,33 @@ struct stt_ctx {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         graph_exec = nullptr;
         graph_batched = -1;
+        ++graph_gen;
+    }
+
+    // Everything stt_iterate checks and prepares before it issues work.
+    int prepare_iteration() {
+        int rc = ready_check();
+        if (rc) return rc;
+        rc = build_tables();
+        if (rc) return rc;
+        if (!pd_ready && (rc = pd_init())) return rc;
+        return STT_OK;
+    }
+};
+
+// Several contexts iterated as one unit (stt_group_*).
+struct stt_group {
+    std::vector<stt_ctx*> ctxs;
+    std::vector<uint64_t> gens;  // the contexts' graph_gen when the graph was captured
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t exec = nullptr;
+
+    ~stt_group() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (fork) (void)hipEventDestroy(fork);
+        for (hipEvent_t e : join)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -1243,11 +1271,8 @@ int stt_pd_extrapolate(stt_ctx* ctx, int frame) {
 int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timing* timing) {
     CTX_CHECK(ctx);
     if (n_iters < 0) return fail(STT_ERR_INVALID, "negative iteration count");
-    int rc = ctx->ready_check();
+    int rc = ctx->prepare_iteration();
     if (rc) return rc;
-    rc = ctx->build_tables();
-    if (rc) return rc;
-    if (!ctx->pd_ready && (rc = ctx->pd_init())) return rc;
     if (timing) std::memset(timing, 0, sizeof(*timing));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) {
@@ -1290,6 +1315,123 @@ int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timin
         timing->total_ms = ms;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    }
+    return STT_OK;
+}
+
+int stt_group_create(stt_ctx* const* ctxs, int n, stt_group** out) {
+    if (!out) return fail(STT_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!ctxs || n < 1) return fail(STT_ERR_INVALID, "a group needs at least one context");
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(STT_ERR_INVALID, "null context in group");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(STT_ERR_INVALID, "group contexts must share one device");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(STT_ERR_INVALID, "a context appears twice in the group");
+    }
+    auto* g = new stt_group();
+    g->ctxs.assign(ctxs, ctxs + n);
+    g->join.assign(n, nullptr);
+    hipError_t e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    for (int i = 1; i < n && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(STT_ERR_HIP, std::string("group events: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return STT_OK;
+}
+
+int stt_group_destroy(stt_group* grp) {
+    delete grp;
+    return STT_OK;
+}
+
+namespace {
+
+// Every context's batched iteration on its own stream, forked from and
+// joined into the first context's stream.
+int issue_group(stt_group* grp) {
+    stt_ctx* c0 = grp->ctxs[0];
+    const int n = (int)grp->ctxs.size();
+    if (n > 1) {
+        HIPCHK(hipEventRecord(grp->fork, c0->stream));
+        for (int i = 1; i < n; ++i) HIPCHK(hipStreamWaitEvent(grp->ctxs[i]->stream, grp->fork, 0));
+    }
+    for (int i = 0; i < n; ++i) {
+        const int rc = grp->ctxs[i]->issue_iteration(true, nullptr);
+        if (rc) return rc;
+    }
+    for (int i = 1; i < n; ++i) {
+        HIPCHK(hipEventRecord(grp->join[i], grp->ctxs[i]->stream));
+        HIPCHK(hipStreamWaitEvent(c0->stream, grp->join[i], 0));
+    }
+    return STT_OK;
+}
+
+}  // namespace
+
+int stt_group_iterate(stt_group* grp, int n_iters, int use_graph, double* total_ms) {
+    if (!grp) return fail(STT_ERR_INVALID, "null group");
+    if (n_iters < 0) return fail(STT_ERR_INVALID, "negative iteration count");
+    for (stt_ctx* c : grp->ctxs) {
+        const int rc = c->prepare_iteration();
+        if (rc) return rc;
+    }
+    stt_ctx* c0 = grp->ctxs[0];
+    // the contexts' own set-up (table uploads, PD initialisation) is on their
+    // streams; the first context's stream waits for it before the group starts
+    for (size_t i = 1; i < grp->ctxs.size(); ++i) {
+        HIPCHK(hipEventRecord(grp->join[i], grp->ctxs[i]->stream));
+        HIPCHK(hipStreamWaitEvent(c0->stream, grp->join[i], 0));
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (total_ms) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c0->stream));
+    }
+    if (use_graph) {
+        bool stale = grp->exec == nullptr;
+        for (size_t i = 0; i < grp->ctxs.size() && !stale; ++i) stale = grp->gens[i] != grp->ctxs[i]->graph_gen;
+        if (stale) {
+            if (grp->exec) (void)hipGraphExecDestroy(grp->exec);
+            grp->exec = nullptr;
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = issue_group(grp);
+            const hipError_t ce = hipStreamEndCapture(c0->stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ce != hipSuccess) return fail(STT_ERR_HIP, std::string("group graph capture: ") + hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&grp->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(STT_ERR_HIP, std::string("group graph instantiate: ") + hipGetErrorString(ie));
+            grp->gens.clear();
+            for (stt_ctx* c : grp->ctxs) grp->gens.push_back(c->graph_gen);
+        }
+        for (int it = 0; it < n_iters; ++it) HIPCHK(hipGraphLaunch(grp->exec, c0->stream));
+    } else {
+        for (int it = 0; it < n_iters; ++it) {
+            const int rc = issue_group(grp);
+            if (rc) return rc;
+        }
+    }
+    if (total_ms) {
+        HIPCHK(hipEventRecord(e1, c0->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *total_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    // later calls on any member context see the group's work finished
+    for (size_t i = 1; i < grp->ctxs.size(); ++i) {
+        HIPCHK(hipEventRecord(grp->fork, c0->stream));
+        HIPCHK(hipStreamWaitEvent(grp->ctxs[i]->stream, grp->fork, 0));
     }
     return STT_OK;
 }

@@ -53,6 +53,14 @@ def test_invalid_arguments_are_rejected_before_device_use():
     assert stt.lib.stt_ctx_create(C.byref(g), C.byref(p), 1, Z, C.byref(out)) == stt.STT_ERR_INVALID
     assert stt.lib.stt_ctx_destroy(None) == stt.STT_ERR_INVALID
     assert stt.lib.stt_iterate(None, 1, 1, 0, None) == stt.STT_ERR_INVALID
+    grp = C.c_void_p()
+    assert stt.lib.stt_group_create(None, 1, C.byref(grp)) == stt.STT_ERR_INVALID
+    none2 = (C.c_void_p * 2)(None, None)
+    assert stt.lib.stt_group_create(none2, 2, C.byref(grp)) == stt.STT_ERR_INVALID
+    assert b"null context" in stt.lib.stt_last_error()
+    assert stt.lib.stt_group_create(none2, 0, C.byref(grp)) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_group_iterate(None, 1, 1, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_group_destroy(None) == stt.STT_OK
 
 
 @pytest.mark.skipif(HAVE_GPU, reason="checks the no-device path")

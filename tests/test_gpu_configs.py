@@ -144,3 +144,41 @@ def test_config4_uneven_batched_iteration():
     print(f"  worst frame relL2 vs oracle {worst:.2e}", flush=True)
     assert worst <= 1e-4
     ctx.close()
+
+
+def _mixed_context(dims, P, seed, det=(96, 80)):
+    g = stt.Geometry.make(dims, det, (1, 1, 1), 1.0, 600.0, 1000.0)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.01, 0.3
+    p.prior = stt.STT_PRIOR_ATV  # SART(1 sweep) + PD ATV, k = e = 0: independent solves
+    ctx = stt.Context(g, p, P)
+    zero = np.zeros(int(np.prod(dims)), np.float32)
+    for f in range(len(P)):
+        ctx.set_views(f, list(stt.synth_angles(2, P[f], offset=7 * f)))
+        ctx.set_volume(f, stt.synth_phantom_frame(dims, f, n_ellipsoids=5, seed=seed, deform_seed=2, amp=1.0))
+        ctx.set_projections(f, np.stack([ctx.forward_project(f, k)[0] for k in range(P[f])]))
+    for f in range(len(P) + 1):
+        ctx.set_volume(f, zero)
+    return ctx
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_config4_mixed_sizes_in_one_group(use_graph):
+    # frames of two grids (scaled-down 128^3 / 192^3 mix) with uneven views,
+    # one context per grid, iterated as one group
+    shapes = [((24, 24, 24), [5, 9, 7]), ((36, 36, 36), [8, 6])]
+    print(f"[config4-mixed] {shapes} use_graph={use_graph}", flush=True)
+    grouped = [_mixed_context(d, P, seed=3 + i) for i, (d, P) in enumerate(shapes)]
+    alone = [_mixed_context(d, P, seed=3 + i) for i, (d, P) in enumerate(shapes)]
+    with stt.Group(grouped) as grp:
+        ms = grp.iterate(2, use_graph=use_graph)
+        ms2 = grp.iterate(1, use_graph=use_graph)  # a replay of the same graph
+    for ctx in alone:
+        ctx.iterate(3, batched=True, use_graph=use_graph)
+    print(f"  group: 2 iterations {ms:.2f} ms, then 1 in {ms2:.2f} ms", flush=True)
+    for (d, P), a, b in zip(shapes, grouped, alone):
+        for f in range(len(P)):
+            assert np.array_equal(a.get_volume(f), b.get_volume(f)), (d, f)
+        assert not np.array_equal(a.get_volume(0), np.zeros(int(np.prod(d)), np.float32))
+    for c in grouped + alone:
+        c.close()
