@@ -84,6 +84,7 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
                     const stt_of_params& p, const int roi[6], hipStream_t s, int* iters, double* residual) {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nb = of_partials(g);
+    const int nbp = of_primal_partials(g, roi);  // <= nb
     DevMem mem;
     // W | G(4) | V'(3) | K4(1) | Kty(3) | y(10) floats, then 2 nb + 2 doubles
     const size_t nf = 22 * n;
@@ -114,7 +115,7 @@ int of_level_device(const VolGeom& g, const float ext[3], const float* I1, const
         hipError_t e = launch_of_primal(mode, vin, vout, y, Kty, G, part, g, roi, s);
         if (e == hipSuccess)
             e = launch_of_dual(mode == 2, first, vout, vin, y, K4, G, part + nb, g, sigma1, hf, p.lambda, huber, s);
-        if (e == hipSuccess && mode == 2) e = launch_sum_partials(part, nb, sums, s);
+        if (e == hipSuccess && mode == 2) e = launch_sum_partials(part, nbp, sums, s);
         if (e == hipSuccess && mode == 2) e = launch_sum_partials(part + nb, nb, sums + 1, s);
         return e;
     };

@@ -1895,9 +1895,11 @@ __global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin
                                                    const float* __restrict__ G, double* __restrict__ partial,
                                                    VolGeom g, OfRoi roi) {
 #pragma clang fp contract(off)
+    // the grid covers the ROI's box in y and z; x keeps whole 64-voxel rows
+    // so that a wave's loads stay aligned to cache lines
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    const int y = roi.y0 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = roi.z0 + blockIdx.z;
     double acc = 0.0;
     if (x < g.nx && y < g.ny && of_in_roi(g, roi, x, y, z)) {
         const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
@@ -2244,6 +2246,18 @@ hipError_t launch_warp(const float* uvw, const float* src, float* out, const Vol
 
 int of_partials(const VolGeom& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
 
+// The primal's grid: 64 x 4 blocks over every x and the ROI's y and z range.
+static dim3 of_primal_grid(const VolGeom& g, const int* roi) {
+    const int ex = g.nx - roi[0] - roi[1], ey = g.ny - roi[2] - roi[3], ez = g.nz - roi[4] - roi[5];
+    if (ex <= 0 || ey <= 0 || ez <= 0) return dim3(0, 0, 0);
+    return dim3((g.nx + 63) / 64, (ey + 3) / 4, ez);
+}
+
+int of_primal_partials(const VolGeom& g, const int* roi) {
+    const dim3 d = of_primal_grid(g, roi);
+    return (int)(d.x * d.y * d.z);
+}
+
 static inline OfRoi of_roi(const int* r) { return OfRoi{r[0], r[1], r[2], r[3], r[4], r[5]}; }
 
 hipError_t launch_of_warp(const float* v, const float* img, float* out, const VolGeom& g, const float* ext,
@@ -2260,7 +2274,8 @@ hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float
 
 hipError_t launch_of_primal(int mode, const float* vin, float* vout, const float* Y, float* kty, const float* G,
                             double* partial, const VolGeom& g, const int* roi, hipStream_t s) {
-    const dim3 grid = vol_grid(g, 1);
+    const dim3 grid = of_primal_grid(g, roi);
+    if (grid.x == 0) return hipSuccess;  // empty ROI: no primal voxel, no partials
     const OfRoi r = of_roi(roi);
     switch (mode) {
         case 1: hipLaunchKernelGGL(k_of_primal<1>, grid, dim3(256), 0, s, vin, vout, Y, kty, G, partial, g, r); break;

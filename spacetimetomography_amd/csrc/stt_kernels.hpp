@@ -48,6 +48,8 @@ hipError_t launch_pi_apply_kt(const float* y, float* w, int F, const VolGeom& g,
 // 2 residual) and dual (res: residual) steps, ROI write-back.  Residual
 // kernels write one FP64 partial per block (of_partials(g) blocks).
 int of_partials(const VolGeom& g);
+// The primal covers the ROI's y and z range only: its number of partials.
+int of_primal_partials(const VolGeom& g, const int* roi);
 hipError_t launch_of_warp(const float* v, const float* img, float* out, const VolGeom& g, const float* ext,
                           hipStream_t s);
 hipError_t launch_of_prep(const float* W, const float* I1, const float* v, float* G, const VolGeom& g,
