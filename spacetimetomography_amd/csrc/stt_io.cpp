@@ -336,6 +336,9 @@ int open_tiff(const char* path, Tiff& t, Fd& f, std::vector<Page>& pages) {
     for (const Page& p : pages)
         if (p.w != p0.w || p.h != p0.h || p.spectrum != p0.spectrum)
             return t.err("pages of different sizes are not supported");
+    // W, H < 2^24, pages < 2^20, channels <= 64: bound the product before it is used
+    const double samples = (double)p0.w * (double)p0.h * (double)pages.size() * (double)p0.spectrum;
+    if (samples > 1099511627776.0) return t.err("image larger than 2^40 samples");
     return STT_OK;
 }
 
@@ -777,6 +780,8 @@ int analyze_header(const char* path, const AnalyzeNames& nm, AnalyzeHeader& ah) 
     for (int k = 0; k < 4; ++k) ah.dims[k] = ndim > k ? s16(42 + 2 * k) : 1;
     if (ndim == 0 || ah.dims[0] == 0 || ah.dims[1] == 0 || ah.dims[2] == 0 || ah.dims[3] == 0)
         return set_error(STT_ERR_INVALID, std::string("Analyze '") + path + "': image with zero dimensions");
+    if ((double)ah.dims[0] * ah.dims[1] * ah.dims[2] * ah.dims[3] > 1099511627776.0)
+        return set_error(STT_ERR_INVALID, std::string("Analyze '") + path + "': image larger than 2^40 samples");
     ah.datatype = s16(70);
     ah.scale = f32(112);
     if (ah.scale == 0.f) ah.scale = 1.f;

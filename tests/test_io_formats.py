@@ -410,6 +410,28 @@ def test_analyze_byte_swapped_header_and_scale(tmp_path):
     assert np.array_equal(got[0], a.astype(np.float32) * np.float32(0.25))
 
 
+def test_oversized_headers_are_rejected(tmp_path):
+    hd = bytearray(348)
+    struct.pack_into("<i", hd, 0, 348)
+    struct.pack_into("<5h", hd, 40, 4, -1, -1, -1, -1)  # 65535^4 samples
+    struct.pack_into("<h", hd, 70, 64)
+    open(tmp_path / "huge.hdr", "wb").write(bytes(hd))
+    open(tmp_path / "huge.img", "wb").write(bytes(8))
+    with pytest.raises(stt.SttError, match="2\\^40"):
+        stt.load_image(str(tmp_path / "huge.hdr"))
+    pages = [np.zeros((1, 4, 4), np.float32)]
+    data = bytearray(_tiff_bytes(pages, 32, 3))
+    # width and length entries (tags 256, 257) to 2^24 - 1 and 2^24 - 1 pages worth: one page of 2^48 samples
+    n = struct.unpack_from("<H", data, 8)[0]
+    for k in range(n):
+        tag = struct.unpack_from("<H", data, 10 + 12 * k)[0]
+        if tag in (256, 257):
+            struct.pack_into("<I", data, 10 + 12 * k + 8, (1 << 24) - 1)
+    p = _write(tmp_path, "huge.tif", bytes(data))
+    with pytest.raises(stt.SttError, match="2\\^40"):
+        stt.load_image(p)
+
+
 def test_analyze_rejects_bad_input(tmp_path):
     with pytest.raises(stt.SttError, match="16-bit"):
         stt.save_analyze(str(tmp_path / "big.hdr"), np.zeros((1, 1, 40000), np.float32))
