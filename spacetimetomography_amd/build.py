@@ -60,7 +60,7 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"])
     os.replace(tmp, LIB)
     tmp = CLI + ".tmp"
-    _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", tmp, os.path.join(CSRC, CLI_SOURCE),
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-pthread", "-o", tmp, os.path.join(CSRC, CLI_SOURCE),
           "-L" + LIBDIR, "-lstt", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, CLI)
     if verbose:

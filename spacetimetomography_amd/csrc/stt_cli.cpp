@@ -38,12 +38,16 @@
 #include <cerrno>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/stt.h"
@@ -325,6 +329,114 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// ------------------------------------------------------------- writer ----
+
+// Background writer of the run's dumps (warped frames, flows, volumes): a file
+// is written while the GPU solves the next frame pair or PD pass, in the order
+// it was queued.  Jobs own their samples; push() blocks while more than
+// kMaxQueued bytes wait, so host memory stays bounded, and written buffers go
+// back to a free list so the next dump reuses their pages.  The first failed
+// write is reported by the next push() or by finish(); later jobs are dropped.
+class Writer {
+public:
+    static constexpr size_t kMaxQueued = size_t(4) << 30;
+
+    Writer() { th_ = std::thread([this] { loop(); }); }
+    ~Writer() { finish(); }
+
+    // a buffer of n floats, recycled when one of that size is free
+    std::vector<float> take(size_t n) {
+        std::lock_guard<std::mutex> lk(m_);
+        for (size_t i = 0; i < free_.size(); ++i)
+            if (free_[i].size() == n) {
+                std::vector<float> v = std::move(free_[i]);
+                free_.erase(free_.begin() + (long)i);
+                free_bytes_ -= n * sizeof(float);
+                return v;
+            }
+        return std::vector<float>(n);
+    }
+
+    // queue one file; false (error printed) once any write has failed
+    bool push(bool tiff, std::string path, std::vector<float> data, const int dims[3], int c) {
+        const size_t bytes = data.size() * sizeof(float);
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return failed_ || queued_ == 0 || queued_ + bytes <= kMaxQueued; });
+        if (failed_) return report();
+        q_.push_back(Job{tiff, std::move(path), std::move(data), {dims[0], dims[1], dims[2]}, c});
+        queued_ += bytes;
+        cv_.notify_all();
+        return true;
+    }
+
+    // wait for every queued file; false (error printed) if a write failed
+    bool finish() {
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            if (!th_.joinable()) return !failed_;
+            stop_ = true;
+            cv_.notify_all();
+        }
+        th_.join();
+        std::lock_guard<std::mutex> lk(m_);
+        return failed_ ? report() : true;
+    }
+
+private:
+    struct Job {
+        bool tiff;
+        std::string path;
+        std::vector<float> data;
+        int d[3];
+        int c;
+    };
+
+    bool report() {
+        if (!reported_) std::fprintf(stderr, "stt-tomography: %s\n", error_.c_str());
+        reported_ = true;
+        return false;
+    }
+
+    void loop() {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+            if (q_.empty()) return;  // stop_ and drained
+            Job j = std::move(q_.front());
+            q_.pop_front();
+            lk.unlock();
+            int rc = STT_OK;
+            std::string msg;
+            if (!failed_) {
+                rc = j.tiff ? stt_save_tiff(j.path.c_str(), j.data.data(), j.d[0], j.d[1], j.d[2], j.c)
+                            : stt_save_analyze(j.path.c_str(), j.data.data(), j.d[0], j.d[1], j.d[2], j.c, nullptr);
+                if (rc != STT_OK) msg = "writing " + j.path + " failed: " + stt_last_error();
+            }
+            lk.lock();
+            if (rc != STT_OK && !failed_) {
+                failed_ = true;
+                error_ = msg;
+            }
+            const size_t bytes = j.data.size() * sizeof(float);
+            queued_ -= bytes;
+            if (free_bytes_ + bytes <= kMaxQueued) {
+                free_bytes_ += bytes;
+                free_.push_back(std::move(j.data));
+            }
+            cv_.notify_all();
+        }
+    }
+
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<Job> q_;
+    std::vector<std::vector<float>> free_;
+    size_t queued_ = 0, free_bytes_ = 0;
+    bool stop_ = false, failed_ = false, reported_ = false;
+    std::string error_;
+    std::thread th_;
+};
+
 // --------------------------------------------------------------- run ----
 
 struct Plan {
@@ -442,10 +554,12 @@ int run(const Config& c, const Plan& pl) {
     if (c.flow && F > 1 && mkdirs(c.flow_dir) != 0) return die("cannot create " + c.flow_dir);
     if (mkdirs(dir_of(c.out)) != 0) return die("cannot create " + dir_of(c.out));
     std::vector<int> idx_sart(F, 1);
-    std::vector<float> i2(V), uvw(3 * V), warped(V), vol(V);
+    std::vector<float> i2(V);
     const int dims[3] = {c.g.vol[0], c.g.vol[1], c.g.vol[2]};
+    Writer out;
     std::vector<int> its((size_t)c.ofp.levels * c.ofp.warps);
     for (int outer = 0; outer < c.outer; ++outer) {
+        const double t_outer = now_s();
         double t_flow = 0.0, t_pd = 0.0, t_dump = 0.0;
         if (c.flow && F > 1) {
             std::printf("==============================================\n"
@@ -456,13 +570,14 @@ int run(const Config& c, const Plan& pl) {
                 API(stt_optical_flow(ctx, f, &c.ofp, its.data()));
                 const double t1 = now_s();
                 t_flow += t1 - t0;
+                std::vector<float> uvw = out.take(3 * V), warped = out.take(V);
                 API(stt_get_buffer(ctx, STT_BUF_FLOW, f, uvw.data()));
                 API(stt_get_volume(ctx, f + 1, i2.data()));
                 API(stt_warp(ctx, uvw.data(), i2.data(), -1.0f, warped.data(), STT_INTERP_CUBIC_CLAMPED));
                 const std::string base = fmt("%s/%03d.%03d.%03d.frame%02d.iter%02d", c.flow_dir.c_str(), dims[0],
                                              dims[1], dims[2], f, outer);
-                API(stt_save_tiff((base + ".tiff").c_str(), warped.data(), dims[0], dims[1], dims[2], 1));
-                API(stt_save_analyze((base + ".field.hdr").c_str(), uvw.data(), dims[0], dims[1], dims[2], 3, nullptr));
+                if (!out.push(true, base + ".tiff", std::move(warped), dims, 1)) return 1;
+                if (!out.push(false, base + ".field.hdr", std::move(uvw), dims, 3)) return 1;
                 t_dump += now_s() - t1;
                 std::printf("flow pair %d: %.3f s, iterations per level/warp:", f, t1 - t0);
                 for (int k : its) std::printf(" %d", k);
@@ -470,7 +585,8 @@ int run(const Config& c, const Plan& pl) {
             }
         }
         if (outer == 0) {
-            std::printf("outer iteration 0: flow %.3f s, warp + dumps %.3f s\n", t_flow, t_dump);
+            std::printf("outer iteration 0: %.3f s wall; flow %.3f s, warp + dump queueing %.3f s\n",
+                        now_s() - t_outer, t_flow, t_dump);
             continue;
         }
         std::printf("==============================================\n"
@@ -496,16 +612,22 @@ int run(const Config& c, const Plan& pl) {
             std::printf("PD pass %d of outer iteration %d: %.3f s\n", pass, outer, t1 - t0);
             // ST_Tomography::SART's dump of each frame after its sweep (:786-805)
             for (int f = 0; f < F; ++f) {
+                std::vector<float> vol = out.take(V);
                 API(stt_get_volume(ctx, f, vol.data()));
                 const std::string name = fmt("%s-%03d-frame-%diter.tiff", c.out.c_str(), f, idx_sart[f]);
-                API(stt_save_tiff(name.c_str(), vol.data(), dims[0], dims[1], dims[2], 1));
                 std::printf("%s\n", name.c_str());
+                if (!out.push(true, name, std::move(vol), dims, 1)) return 1;
                 ++idx_sart[f];
             }
             t_dump += now_s() - t1;
         }
-        std::printf("outer iteration %d: flow %.3f s, PD %.3f s, warp + dumps %.3f s\n", outer, t_flow, t_pd, t_dump);
+        std::printf("outer iteration %d: %.3f s wall; flow %.3f s, PD %.3f s, warp + dump queueing %.3f s\n", outer,
+                    now_s() - t_outer, t_flow, t_pd, t_dump);
     }
+    // the files of the last iteration have no GPU work left to hide behind
+    const double t_drain = now_s();
+    if (!out.finish()) return 1;
+    std::printf("remaining dumps written in %.3f s\n", now_s() - t_drain);
     return 0;
 }
 

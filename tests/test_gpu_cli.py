@@ -79,7 +79,7 @@ def _vol(path):
 def test_cli_render_equals_in_process_sequence(tmp_path):
     g, stack, init = _scene(tmp_path)
     out = _cli(tmp_path, "-s", "0.2", "-t", "0.5", "-b", "2", "-p", "1")
-    assert "done in" in out
+    assert "done in" in out and "remaining dumps written in" in out
     ctx = _expected_context(g, stack, init, 0.2, 0.5)
     ofp = _of_params()
     base = "{:03d}.{:03d}.{:03d}.frame00.iter{:02d}".format(*DIMS, 0)
@@ -118,3 +118,17 @@ def test_cli_nested_sart_sweeps_equal_step_sequence(tmp_path):
         assert np.array_equal(_vol(tmp_path / "out" / f"rec-{f:03d}-frame-1iter.tiff"), ctx.get_volume(f))
     assert not os.path.exists(tmp_path / "FlowOutput")
     ctx.close()
+
+
+def test_cli_reports_failed_dump(tmp_path):
+    """A file the background writer cannot create fails the run with its path."""
+    _scene(tmp_path)
+    os.makedirs(tmp_path / "out" / "rec-001-frame-1iter.tiff")  # a directory where a dump goes
+    cmd = [CLI, "-i", "proj.tif", "-f", str(DIMS[0]), "-f", str(DIMS[1]), "-f", str(DIMS[2]), "-u", str(DET[0]),
+           "-u", str(DET[1]), "--oxyz", "0", "--sdg", "0", "-v", "2", "-r", "1", "-o", "out/rec",
+           "--joint-iters", "2", "-b", "2", "--no-flow"]
+    print("cli:", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert p.returncode == 1, p.stdout
+    assert "rec-001-frame-1iter.tiff failed" in p.stdout
+    assert "done in" not in p.stdout
