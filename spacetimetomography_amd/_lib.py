@@ -399,6 +399,17 @@ class Context:
         check(lib.stt_get_buffer(self.h, which, frame, fptr(out)))
         return out
 
+    def set_buffer(self, which, frame, data):
+        n = 4 * self.V if which == STT_BUF_YTV else (3 * self.V if which == STT_BUF_FLOW else self.V)
+        a = np.ascontiguousarray(data, np.float32).reshape(-1)
+        if a.size != n:
+            raise ValueError(f"buffer {which} holds {n} floats, got {a.size}")
+        check(lib.stt_set_buffer(self.h, which, frame, fptr(a)))
+
+    def set_params(self, params):
+        """stt_ctx_set_params: new PD/SART/warp parameters (the backprojector stays)."""
+        check(lib.stt_ctx_set_params(self.h, C.byref(params)))
+
     def gmax(self):
         g = C.c_float()
         check(lib.stt_ctx_get_gmax(self.h, C.byref(g)))

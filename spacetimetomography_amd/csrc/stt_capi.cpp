@@ -405,6 +405,17 @@ struct stt_ctx {
     std::vector<float*> x, xT, xbar, xprev, ytv, ytp, ybc, flow, b, raylen_f, corr_f, contrib_f, wsum_f;
     float* raylen_shared = nullptr;
     float* zero_flow = nullptr;
+    // End points of the PD warps' traces of each flow u_f (warping.h:100-114):
+    // trace_bw[f] with step +1 for the dual's backward warp of X_bar[f+1],
+    // trace_fw[f] with step -1 for frame f+1's forward warp of Y_bc[f].  They
+    // depend on the flow and the interpolation only, which stay fixed over the
+    // PD passes of an outer iteration, so the velocity sampling runs once per
+    // flow instead of in every pass.  flow_gen[f] counts the writes of u_f;
+    // empty vectors (allocation refused) mean the kernels trace themselves.
+    std::vector<float*> trace_bw, trace_fw;
+    std::vector<uint64_t> flow_gen, trace_gen;
+    int trace_interp = -1;
+    bool trace_tried = false;
     ViewRec* views_dev = nullptr;
     std::vector<ViewRec> views_host;
     std::vector<char> views_set;
@@ -551,6 +562,8 @@ struct stt_ctx {
             fbs[f].ytp = ytp[f];
             fbs[f].ybc = ybc[f];
             fbs[f].flow = f < F ? flow[f] : zero_flow;
+            fbs[f].trace_bw = f < (int)trace_bw.size() ? trace_bw[f] : nullptr;
+            fbs[f].trace_fw = f < (int)trace_fw.size() ? trace_fw[f] : nullptr;
         }
         HIPCHK(hipMemcpyAsync(fb_dev, fbs.data(), fbs.size() * sizeof(FrameBufs), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -704,6 +717,48 @@ struct stt_ctx {
         return STT_OK;
     }
 
+    void flow_changed(int f) { ++flow_gen[f]; }
+
+    // Brings the stored trace end points up to date with the flows and the
+    // interpolation (stream-ordered before the PD steps that read them).  The
+    // first call allocates them, 24 bytes per voxel and frame; if the device
+    // refuses that memory, the PD kernels keep tracing in place.
+    int refresh_traces() {
+        if (!trace_tried) {
+            trace_tried = true;
+            std::vector<float*> bw(F, nullptr), fw(F, nullptr);
+            bool ok = true;
+            for (int f = 0; f < F && ok; ++f) {
+                void* a = nullptr;
+                void* b2 = nullptr;
+                ok = hipMalloc(&a, 3 * V * sizeof(float)) == hipSuccess;
+                if (ok) {
+                    allocs.push_back(a);
+                    ok = hipMalloc(&b2, 3 * V * sizeof(float)) == hipSuccess;
+                    if (ok) allocs.push_back(b2);
+                }
+                bw[f] = (float*)a;
+                fw[f] = (float*)b2;
+            }
+            if (ok) {
+                trace_bw = bw;
+                trace_fw = fw;
+                tables_dirty = true;
+            } else {
+                (void)hipGetLastError();  // the refused allocation is not an error of the next call
+            }
+        }
+        if (!trace_bw.empty()) {
+            for (int f = 0; f < F; ++f) {
+                if (trace_gen[f] == flow_gen[f] && trace_interp == pc.interp) continue;
+                HIPCHK(launch_trace(flow[f], trace_bw[f], trace_fw[f], vg, pc.interp, stream));
+                trace_gen[f] = flow_gen[f];
+            }
+            trace_interp = pc.interp;
+        }
+        return build_tables();
+    }
+
     void invalidate_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         graph_exec = nullptr;
@@ -715,7 +770,7 @@ struct stt_ctx {
     int prepare_iteration() {
         int rc = ready_check();
         if (rc) return rc;
-        rc = build_tables();
+        rc = refresh_traces();
         if (rc) return rc;
         if (!pd_ready && (rc = pd_init())) return rc;
         return STT_OK;
@@ -901,6 +956,8 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     if ((rc = c->alloc_t(&c->items_scratch, (size_t)c->scratch_cap))) return bail(rc);
     c->views_host.resize(c->total_views);
     c->views_set.assign(F, 0);
+    c->flow_gen.assign(F, 1);
+    c->trace_gen.assign(F, 0);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(STT_ERR_HIP, "stream sync failed"));
     *out = c;
     return STT_OK;
@@ -999,6 +1056,7 @@ int stt_set_buffer(stt_ctx* ctx, int which, int frame, const float* in) {
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(dst, in, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (which == STT_BUF_FLOW) ctx->flow_changed(frame);
     return STT_OK;
 }
 
@@ -1044,6 +1102,7 @@ int stt_set_flow(stt_ctx* ctx, int pair, const float* uvw) {
     if (pair < 0 || pair >= ctx->F) return fail(STT_ERR_INVALID, "flow pair out of range");
     HIPCHK(hipMemcpyAsync(ctx->flow[pair], uvw, 3 * ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->flow_changed(pair);
     return STT_OK;
 }
 
@@ -1244,7 +1303,7 @@ int stt_pd_init(stt_ctx* ctx) {
 int stt_pd_dual(stt_ctx* ctx, int frame) {
     CTX_CHECK(ctx);
     if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
-    int rc = ctx->build_tables();
+    int rc = ctx->refresh_traces();
     if (rc) return rc;
     HIPCHK(launch_pd_dual(ctx->fb_dev, frame, 1, ctx->vg, ctx->pc, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1254,7 +1313,7 @@ int stt_pd_dual(stt_ctx* ctx, int frame) {
 int stt_pd_primal(stt_ctx* ctx, int frame) {
     CTX_CHECK(ctx);
     if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
-    int rc = ctx->build_tables();
+    int rc = ctx->refresh_traces();
     if (rc) return rc;
     HIPCHK(launch_pd_primal(ctx->fb_dev, frame, 1, ctx->vg, ctx->pc, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1766,6 +1825,7 @@ int stt_optical_flow(stt_ctx* ctx, int pair, const stt_of_params* p, int* iters)
     rc = check_dims(dims, 4);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->flow_changed(pair);  // also when the solve fails part-way: the buffer may hold a partial flow
     return of_multiscale_device(dims, ctx->x[pair], ctx->x[pair + 1], ctx->flow[pair], *p, ctx->stream, iters);
 }
 

@@ -83,8 +83,9 @@ struct FrameBufs {
     float* ytv;   // 3 planes
     float* ytp;
     float* ybc;
-    float* flow;  // u_f, 3 planes (zero for the last frame)
-    float* pad;
+    float* flow;      // u_f, 3 planes (zero for the last frame)
+    float* trace_bw;  // end points of u_f traced by +1 (dual's warp), 3 planes; null: trace in the kernel
+    float* trace_fw;  // end points of u_f traced by -1 (next frame's primal warp), 3 planes; or null
 };
 
 }  // namespace stt

@@ -1387,14 +1387,14 @@ __device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned 
 }
 
 // trace (warping.h:100-114) from the cell centre of (i,j,k) with trace step
-// tdt (= -dt of warp), then the image sample.  The first velocity sample sits
-// exactly on grid node clamp(i,1,n-2), where the clamped cubic returns the node
-// value, so it is read directly.  The three velocity planes share one buffer.
+// tdt (= -dt of warp): the end point q at which warp samples the image.  The
+// first velocity sample sits exactly on grid node clamp(i,1,n-2), where the
+// clamped cubic returns the node value, so it is read directly.  The three
+// velocity planes share one buffer.
 template <bool CUBIC>
-__device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, const VolGeom& g, int i, int j, int k,
-                                            float tdt) {
+__device__ __forceinline__ void trace_voxel(const float* uvw, const VolGeom& g, int i, int j, int k, float tdt,
+                                            float q[3]) {
     const unsigned V = (unsigned)(g.nx * g.ny * g.nz);
-    const __amdgpu_buffer_rsrc_t rs_img = __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, (int)(4u * V), kBufWord3);
     const __amdgpu_buffer_rsrc_t rs_vel =
         __builtin_amdgcn_make_buffer_rsrc((void*)uvw, (short)0, (int)(12u * V), kBufWord3);
     const int ci = min(max(i, 1), g.nx - 2), cj = min(max(j, 1), g.ny - 2), ck = min(max(k, 1), g.nz - 2);
@@ -1410,14 +1410,69 @@ __device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, 
     float v2[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) v2[c] = sample_img<CUBIC>(rs_vel, 4u * V * (unsigned)c, s);
-    float q[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float p0 = p2[c] - tdt * v2[c];
         q[c] = p2[c] + 0.5f * (p[c] - p0);
     }
+}
+
+// The image sample of warp at a traced end point q.
+template <bool CUBIC>
+__device__ __forceinline__ float sample_at(const float* img, const VolGeom& g, const float q[3]) {
+    const unsigned V = (unsigned)(g.nx * g.ny * g.nz);
+    const __amdgpu_buffer_rsrc_t rs_img = __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, (int)(4u * V), kBufWord3);
+    SamplePt s;
     sample_point<CUBIC>(g, q[0], q[1], q[2], s);
     return sample_img<CUBIC>(rs_img, 0u, s);
+}
+
+// warp of img by uvw at voxel (i,j,k): trace, then sample.
+template <bool CUBIC>
+__device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, const VolGeom& g, int i, int j, int k,
+                                            float tdt) {
+    float q[3];
+    trace_voxel<CUBIC>(uvw, g, i, j, k, tdt, q);
+    return sample_at<CUBIC>(img, g, q);
+}
+
+// The PD steps' warp of img at voxel (i,j,k): from the stored end points of
+// the trace when the context keeps them (three planes of V floats), else
+// traced here.  Both give the same float end point, so the same sample.
+__device__ __forceinline__ float pd_warp(const float* img, const float* uvw, const float* tr, const VolGeom& g,
+                                         int interp, int i, int j, int k, size_t qi, float tdt) {
+    if (tr) {
+        const size_t V = (size_t)g.nx * g.ny * g.nz;
+        const gfloat* t = gptr(tr);
+        const float q[3] = {t[qi], t[V + qi], t[2 * V + qi]};
+        return interp == 0 ? sample_at<true>(img, g, q) : sample_at<false>(img, g, q);
+    }
+    return interp == 0 ? warp_voxel<true>(img, uvw, g, i, j, k, tdt) : warp_voxel<false>(img, uvw, g, i, j, k, tdt);
+}
+
+// End points of the two PD traces of flow u_f at every voxel: step +1 (the
+// dual's backward warp, warp(u_f, ., dt = -1)) into tb, step -1 (the primal's
+// forward warp, dt = +1) into tf; three planes each.
+template <bool CUBIC>
+__global__ __launch_bounds__(256) void k_trace(const float* __restrict__ uvw, VolGeom g, float* __restrict__ tb,
+                                              float* __restrict__ tf) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const size_t V = (size_t)g.nx * g.ny * g.nz;
+    const size_t qi = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+    float q[3];
+    trace_voxel<CUBIC>(uvw, g, x, y, z, 1.0f, q);
+    gfloat* b = gptr(tb);
+    b[qi] = q[0];
+    b[V + qi] = q[1];
+    b[2 * V + qi] = q[2];
+    trace_voxel<CUBIC>(uvw, g, x, y, z, -1.0f, q);
+    gfloat* f = gptr(tf);
+    f[qi] = q[0];
+    f[V + qi] = q[1];
+    f[2 * V + qi] = q[2];
 }
 
 __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, const float* __restrict__ src,
@@ -1448,8 +1503,7 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
     gfloat* Ytp = gptr(B.ytp);
     gfloat* Ybc = gptr(B.ybc);
     // m_backward_warp = warp(u_f, X_bar[f+1], dt = -1): trace step +1 (:503)
-    const float bw = pc.interp == 0 ? warp_voxel<true>(B1.xbar, B.flow, g, x, y, z, 1.0f)
-                                    : warp_voxel<false>(B1.xbar, B.flow, g, x, y, z, 1.0f);
+    const float bw = pd_warp(B1.xbar, B.flow, B.trace_bw, g, pc.interp, x, y, z, q, 1.0f);
     const float xq = xb0[q];
     float ytp = Ytp[q] + pc.sigma * (xb1[q] - xq);  // (:504)
     float ybc = Ybc[q] + pc.sigma * (bw - xq);      // (:505)
@@ -1504,8 +1558,7 @@ __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__
         xv = xv - pc.tau * (-Ybc[q]);
     } else {                // (:546-555): forward warp of Y_bc[f-1] with u_{f-1}, trace step -1
         const FrameBufs& Bm = fb[f - 1];
-        const float fw = pc.interp == 0 ? warp_voxel<true>(Bm.ybc, Bm.flow, g, x, y, z, -1.0f)
-                                        : warp_voxel<false>(Bm.ybc, Bm.flow, g, x, y, z, -1.0f);
+        const float fw = pd_warp(Bm.ybc, Bm.flow, Bm.trace_fw, g, pc.interp, x, y, z, q, -1.0f);
         xv = xv - pc.tau * (gptr(Bm.ytp)[q] - Ytp[q]);
         xv = xv - pc.tau * (fw - Ybc[q]);
     }
@@ -2185,6 +2238,14 @@ hipError_t launch_pd_dual(const FrameBufs* fb, int f0, int nf, const VolGeom& g,
 
 hipError_t launch_pd_primal(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s) {
     hipLaunchKernelGGL(k_pd_primal, vol_grid(g, nf), dim3(256), 0, s, fb, f0, g, pc);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const float* uvw, float* tb, float* tf, const VolGeom& g, int interp, hipStream_t s) {
+    if (interp == 0)
+        hipLaunchKernelGGL(k_trace<true>, vol_grid(g, 1), dim3(256), 0, s, uvw, g, tb, tf);
+    else
+        hipLaunchKernelGGL(k_trace<false>, vol_grid(g, 1), dim3(256), 0, s, uvw, g, tb, tf);
     return hipGetLastError();
 }
 

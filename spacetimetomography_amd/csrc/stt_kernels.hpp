@@ -23,6 +23,8 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s);
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_bp_ray_apply(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
+// End points of the PD traces of one flow (step +1 into tb, -1 into tf).
+hipError_t launch_trace(const float* uvw, float* tb, float* tf, const VolGeom& g, int interp, hipStream_t s);
 hipError_t launch_pd_dual(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_pd_primal(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s);
 hipError_t launch_extrapolate(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc,

@@ -307,6 +307,66 @@ def test_st_iteration_matches_oracle(mode, prior):
     ctx.close()
 
 
+def test_flow_change_between_iterations_matches_oracle():
+    """The PD warps' stored trace end points follow each flow write: after new
+    flows (one through stt_set_flow, one through stt_set_buffer) the next
+    iteration equals the oracle's on the new flows."""
+    print("[trace-refresh] F=3 26^3 det 36x36, flows replaced after one iteration", flush=True)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    P = (5, 6, 4)
+    vol = (26, 26, 26)
+    ga = lambda f: list(stt.synth_angles(1, P[f], offset=sum(P[:f])))
+    g, ctx, oc = _setup_pair(vol, (36, 36), P, ga, p, F=3)
+    oc.per_frame_raylen(True)
+    ctx.iterate(1, batched=True, use_graph=True)
+    oc.iterate(1)
+    u0 = stt.synth_flow(vol, 4, deform_seed=9, amp=1.5)
+    u1 = stt.synth_flow(vol, 5, deform_seed=9, amp=2.5)
+    ctx.set_flow(0, u0)
+    oc.set_flow(0, u0)
+    ctx.set_buffer(stt.STT_BUF_FLOW, 1, u1)
+    oc.set_flow(1, u1)
+    ctx.iterate(1, batched=True, use_graph=True)
+    oc.iterate(1)
+    for f in range(3):
+        got, ref = ctx.get_volume(f), oc.get_volume(f)
+        assert rel_l2(got, ref) <= 1e-4, (f, rel_l2(got, ref))
+        assert rel_l2(ctx.get_buffer(stt.STT_BUF_YBC, f), oc.get_buffer(stt.STT_BUF_YBC, f)) <= 1e-4
+    ctx.close()
+
+
+def test_interp_change_recomputes_traces():
+    """Switching the warp interpolation with stt_ctx_set_params re-traces the
+    flows: the PD steps then equal the oracle's run with the new interpolation."""
+    print("[trace-interp] F=3 24x22x26, cubic traces, then trilinear steps", flush=True)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01
+    p.interp = stt.STT_INTERP_CUBIC_CLAMPED
+    vol = (24, 22, 26)
+    g, ctx, _ = _setup_pair(vol, (32, 32), (4, 4, 4), lambda f: [0, 90, 180, 270], p, F=3)
+    ctx.iterate(0)  # PD initialisation and the cubic traces, no step
+    p2 = stt.params_default()
+    p2.sigma, p2.tau, p2.lambda_tv, p2.k_tp, p2.e_bc, p2.huber = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01
+    p2.interp = stt.STT_INTERP_TRILINEAR
+    ctx.set_params(p2)
+    g2, ctx2, oc = _setup_pair(vol, (32, 32), (4, 4, 4), lambda f: [0, 90, 180, 270], p2, F=3)
+    ctx2.close()
+    oc.pd_init()
+    for f in range(3):
+        ctx.pd_dual(f)
+        oc.pd_dual(f)
+    for f in range(3):
+        ctx.pd_primal(f)
+        oc.pd_primal(f)
+    for f in range(3):
+        for which in (stt.STT_BUF_YBC, stt.STT_BUF_YTV):
+            a, b = ctx.get_buffer(which, f), oc.get_buffer(which, f)
+            assert rel_l2(a, b) <= 1e-5, (f, which, rel_l2(a, b))
+        assert rel_l2(ctx.get_volume(f), oc.get_volume(f)) <= 1e-5
+    ctx.close()
+
+
 def test_batched_equals_plain():
     print("[batched-vs-plain] F=4", flush=True)
     p = stt.params_default()
