@@ -129,7 +129,12 @@ int  stt_set_view_quaternions(stt_ctx* ctx, int frame, const float* q_xyzw, int 
  * frame == nframes is the padding copy. */
 int  stt_set_volume(stt_ctx* ctx, int frame, const float* vol);
 int  stt_get_volume(stt_ctx* ctx, int frame, float* vol);
-int  stt_set_flow(stt_ctx* ctx, int pair, const float* uvw);       /* pair in [0, nframes-1] */
+/* pair in [0, nframes-1].  The context keeps the end points of the PD warps'
+ * traces of each flow (24 bytes per voxel and frame, allocated on the first
+ * PD step; without that memory the kernels trace in place).  Every write of a
+ * flow -- stt_set_flow, stt_set_buffer(STT_BUF_FLOW), stt_optical_flow -- and
+ * every change of p.interp makes the next PD step re-trace it. */
+int  stt_set_flow(stt_ctx* ctx, int pair, const float* uvw);
 int  stt_set_projections(stt_ctx* ctx, int frame, const float* b); /* W*H*views[frame] */
 /* Device-pointer variants (borrowed, copied device-to-device). */
 int  stt_set_volume_dev(stt_ctx* ctx, int frame, const float* vol_dev);
