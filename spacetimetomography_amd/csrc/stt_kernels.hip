@@ -626,6 +626,7 @@ constexpr int kSlabK = 23;      // most sample planes per slab (box depth <= 24)
 constexpr int kSlabKMin = 8;    // fewest sample planes per slab (before the LDS cap)
 constexpr float kSlabShear = 0.5f;  // slab depth: the box may widen by this fraction of its near-plane width
 constexpr int kStagePasses = 4; // staging covers x-y windows up to 4*256 elements
+constexpr int kStageBatch = 4;  // planes loaded per staging round
 
 struct SlabBox {
     int lo[3], n[3];   // box origin and extent per volume axis (x, y, z)
@@ -730,8 +731,8 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
 
 // Cooperative load of the box into LDS.  Each thread owns fixed positions
 // (column, row) of the box's x-y window -- element tid + NT*m of it -- and
-// walks the z planes, so a plane step moves only wave-uniform offsets.  Up
-// to 8 loads are in flight before their LDS writes.  Indices up to dims (the
+// walks the z planes, so a plane step moves only wave-uniform offsets.
+// kStageBatch loads are in flight before their LDS writes.  Indices up to dims (the
 // +1 taps of the last voxel, weight 0) read the volume's zero tail or the
 // next row, never past the allocation.
 template <int NT>
@@ -752,16 +753,18 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
         const unsigned voff = 4u * ((unsigned)bx.lo[0] + col + nx * ((unsigned)bx.lo[1] + row));
         const int loff = (int)col + bx.px * (int)row;
         if (valid) {
-            for (int z0 = 0; z0 < nbz; z0 += 8) {
-                float vals[8];
+            // kStageBatch planes per round, branch-free: the last round
+            // repeats plane nbz-1 instead of testing each plane (uniform
+            // branches around every load and write cost 1.2 ms per
+            // iteration at 256^3 x 8; batches of 3, 5-8 and 12 measured slower)
+            for (int z0 = 0; z0 < nbz; z0 += kStageBatch) {
+                float vals[kStageBatch];
 #pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (z0 + k < nbz)
-                        vals[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                                vv.rsrc, voff, zbase + 4u * nxy * (unsigned)(z0 + k), 0));
+                for (int k = 0; k < kStageBatch; ++k)
+                    vals[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                            vv.rsrc, voff, zbase + 4u * nxy * (unsigned)min(z0 + k, nbz - 1), 0));
 #pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (z0 + k < nbz) box[loff + bx.pxy * (z0 + k)] = vals[k];
+                for (int k = 0; k < kStageBatch; ++k) box[loff + bx.pxy * min(z0 + k, nbz - 1)] = vals[k];
             }
         }
     }
