@@ -741,7 +741,9 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
     const int tid = threadIdx.x;
     const int nbx = bx.n[0], nby = bx.n[1], nbz = bx.n[2];
     const int npl = nbx * nby;
-    const unsigned magic = (unsigned)(0xFFFFFFFFull / (unsigned)nbx) + 1u;  // e / nbx for e < 2^16
+    // e / nbx for e < 2^16; a one-column box (no sample of the block in the
+    // slab) would overflow the multiplier, so its rows are taken directly
+    const unsigned magic = nbx > 1 ? (unsigned)(0xFFFFFFFFull / (unsigned)nbx) + 1u : 0u;
     const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
     const unsigned zbase = 4u * nxy * (unsigned)bx.lo[2];
 #pragma unroll
@@ -749,7 +751,7 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
         if (NT * m >= npl) break;  // uniform
         const int e = tid + NT * m;
         const bool valid = e < npl;
-        const unsigned row = __umulhi((unsigned)e, magic);
+        const unsigned row = nbx > 1 ? __umulhi((unsigned)e, magic) : (unsigned)e;
         const unsigned col = (unsigned)e - row * (unsigned)nbx;
         const unsigned voff = 4u * ((unsigned)bx.lo[0] + col + nx * ((unsigned)bx.lo[1] + row));
         const int loff = (int)col + bx.px * (int)row;
