@@ -643,7 +643,8 @@ struct stt_ctx {
             // backprojection overlaps another group's projector.  The duals
             // and primals of all frames run first, as one launch each (a
             // group's primal run beside the other groups' sweeps made the
-            // iteration 0.7% slower).  Joined before returning.
+            // iteration 0.7% slower, and no faster once the primal read
+            // stored trace end points).  Joined before returning.
             HIPCHK(launch_pd_dual(fb_dev, 0, F, vg, pc, stream));
             HIPCHK(launch_pd_primal(fb_dev, 0, F, vg, pc, stream));
             HIPCHK(hipEventRecord(ev_fork, stream));
