@@ -727,26 +727,23 @@ struct stt_ctx {
     int refresh_traces() {
         if (!trace_tried) {
             trace_tried = true;
-            std::vector<float*> bw(F, nullptr), fw(F, nullptr);
+            std::vector<void*> got;
             bool ok = true;
-            for (int f = 0; f < F && ok; ++f) {
+            for (int f = 0; f < 2 * F && ok; ++f) {
                 void* a = nullptr;
-                void* b2 = nullptr;
                 ok = hipMalloc(&a, 3 * V * sizeof(float)) == hipSuccess;
-                if (ok) {
-                    allocs.push_back(a);
-                    ok = hipMalloc(&b2, 3 * V * sizeof(float)) == hipSuccess;
-                    if (ok) allocs.push_back(b2);
-                }
-                bw[f] = (float*)a;
-                fw[f] = (float*)b2;
+                if (ok) got.push_back(a);
             }
             if (ok) {
-                trace_bw = bw;
-                trace_fw = fw;
+                for (int f = 0; f < F; ++f) {
+                    trace_bw.push_back((float*)got[2 * f]);
+                    trace_fw.push_back((float*)got[2 * f + 1]);
+                }
+                allocs.insert(allocs.end(), got.begin(), got.end());
                 tables_dirty = true;
             } else {
-                (void)hipGetLastError();  // the refused allocation is not an error of the next call
+                for (void* a : got) (void)hipFree(a);  // none of it is kept
+                (void)hipGetLastError();              // the refused allocation is not an error of the next call
             }
         }
         if (!trace_bw.empty()) {
