@@ -186,6 +186,11 @@ int  stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n);
  * on the context's volume grid; host pointers; aabb = [0, dims]. */
 int  stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt,
               float* out, int interp);
+/* The same warp on the context's own buffers: out (host, V floats) =
+ * warp(flow[pair], volume[pair + 1], dt).  Render dumps this after each flow
+ * solve, I2 warped back by its flow with dt = -1 (ST_Tomography.cpp:462);
+ * here the flow and the frame stay on the device.  pair in [0, nframes-2]. */
+int  stt_warp_frame(stt_ctx* ctx, int pair, float dt, int interp, float* out);
 /* VJPs of the trilinear warp (stt_warp with STT_INTERP_TRILINEAR): given
  * g = dL/d(out), accumulate dL/d(src) into gsrc and dL/d(uvw) into guvw
  * (host arrays, either may be NULL).  The clamps of the sampling grid to

@@ -108,6 +108,7 @@ _SIGS = {
     "stt_fp_vjp": (C.c_int, [_P, C.c_int, C.c_int, _F, _F]),
     "stt_sart_sweep": (C.c_int, [_P, C.c_int, _I, C.c_int]),
     "stt_warp": (C.c_int, [_P, _F, _F, C.c_float, _F, C.c_int]),
+    "stt_warp_frame": (C.c_int, [_P, C.c_int, C.c_float, C.c_int, _F]),
     "stt_warp_vjp": (C.c_int, [_P, _F, _F, C.c_float, _F, _F, _F]),
     "stt_pd_init": (C.c_int, [_P]),
     "stt_pd_dual": (C.c_int, [_P, C.c_int]),
@@ -446,6 +447,12 @@ class Context:
         out = np.empty(self.V, np.float32)
         check(lib.stt_warp(self.h, fptr(np.ascontiguousarray(uvw, np.float32)),
                            fptr(np.ascontiguousarray(src, np.float32)), dt, fptr(out), interp))
+        return out
+
+    def warp_frame(self, pair, dt=-1.0, interp=STT_INTERP_CUBIC_CLAMPED):
+        """stt_warp_frame: volume pair+1 warped by flow pair, on the device buffers."""
+        out = np.empty(self.V, np.float32)
+        check(lib.stt_warp_frame(self.h, pair, dt, interp, fptr(out)))
         return out
 
     def optical_flow(self, pair, params):

@@ -1267,6 +1267,23 @@ int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* 
     return STT_OK;
 }
 
+int stt_warp_frame(stt_ctx* ctx, int pair, float dt, int interp, float* out) {
+    CTX_CHECK(ctx);
+    if (!out) return fail(STT_ERR_INVALID, "null output");
+    if (pair < 0 || pair + 1 >= ctx->F) return fail(STT_ERR_INVALID, "frame pair out of range");
+    if (interp != STT_INTERP_CUBIC_CLAMPED && interp != STT_INTERP_TRILINEAR)
+        return fail(STT_ERR_INVALID, "unknown interpolation");
+    const size_t V = ctx->V;
+    float* buf = nullptr;
+    HIPCHK(hipMallocAsync((void**)&buf, V * sizeof(float), ctx->stream));
+    HIPCHK(launch_warp(ctx->flow[pair], ctx->x[pair + 1], buf, ctx->vg, dt, interp == STT_INTERP_CUBIC_CLAMPED,
+                       ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, buf, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(buf, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
 int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, const float* g, float* gsrc,
                  float* guvw) {
     CTX_CHECK(ctx);

@@ -554,7 +554,6 @@ int run(const Config& c, const Plan& pl) {
     if (c.flow && F > 1 && mkdirs(c.flow_dir) != 0) return die("cannot create " + c.flow_dir);
     if (mkdirs(dir_of(c.out)) != 0) return die("cannot create " + dir_of(c.out));
     std::vector<int> idx_sart(F, 1);
-    std::vector<float> i2(V);
     const int dims[3] = {c.g.vol[0], c.g.vol[1], c.g.vol[2]};
     Writer out;
     std::vector<int> its((size_t)c.ofp.levels * c.ofp.warps);
@@ -572,8 +571,8 @@ int run(const Config& c, const Plan& pl) {
                 t_flow += t1 - t0;
                 std::vector<float> uvw = out.take(3 * V), warped = out.take(V);
                 API(stt_get_buffer(ctx, STT_BUF_FLOW, f, uvw.data()));
-                API(stt_get_volume(ctx, f + 1, i2.data()));
-                API(stt_warp(ctx, uvw.data(), i2.data(), -1.0f, warped.data(), STT_INTERP_CUBIC_CLAMPED));
+                // warping::warp(dim, aabb, uvw, I2, -dt) on the device copies (:462)
+                API(stt_warp_frame(ctx, f, -1.0f, STT_INTERP_CUBIC_CLAMPED, warped.data()));
                 const std::string base = fmt("%s/%03d.%03d.%03d.frame%02d.iter%02d", c.flow_dir.c_str(), dims[0],
                                              dims[1], dims[2], f, outer);
                 if (!out.push(true, base + ".tiff", std::move(warped), dims, 1)) return 1;

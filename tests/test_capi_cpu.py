@@ -61,6 +61,7 @@ def test_invalid_arguments_are_rejected_before_device_use():
     assert stt.lib.stt_group_create(none2, 0, C.byref(grp)) == stt.STT_ERR_INVALID
     assert stt.lib.stt_group_iterate(None, 1, 1, None) == stt.STT_ERR_INVALID
     assert stt.lib.stt_group_destroy(None) == stt.STT_OK
+    assert stt.lib.stt_warp_frame(None, 0, -1.0, 0, None) == stt.STT_ERR_INVALID
 
 
 @pytest.mark.skipif(HAVE_GPU, reason="checks the no-device path")

@@ -190,6 +190,28 @@ def test_warp_matches_oracle(interp, dt):
     ctx.close()
 
 
+@pytest.mark.parametrize("interp", [stt.STT_INTERP_CUBIC_CLAMPED, stt.STT_INTERP_TRILINEAR])
+def test_warp_frame_equals_host_warp(interp):
+    """stt_warp_frame on the context's flow and frame equals stt_warp on
+    the same arrays passed from the host, bit for bit; bad pairs are refused."""
+    print(f"[warp-frame] interp={interp} 26x22x18 F=3", flush=True)
+    dims = (26, 22, 18)
+    g = stt.Geometry.make(dims, (32, 32), (1, 1, 1), 1.0, 60.0, 100.0)
+    ctx = stt.Context(g, stt.params_default(), [2, 2, 2])
+    for f in range(3):
+        ctx.set_volume(f, stt.synth_phantom_frame(dims, f, n_ellipsoids=5, seed=4, deform_seed=5, amp=2.0))
+    for f in range(2):
+        ctx.set_flow(f, stt.synth_flow(dims, f, deform_seed=5, amp=2.0))
+    for pair in range(2):
+        got = ctx.warp_frame(pair, -1.0, interp)
+        ref = ctx.warp(ctx.get_buffer(stt.STT_BUF_FLOW, pair), ctx.get_volume(pair + 1), -1.0, interp)
+        assert np.array_equal(got, ref), pair
+    for bad in (-1, 2):
+        with pytest.raises(stt.SttError):
+            ctx.warp_frame(bad)
+    ctx.close()
+
+
 def test_warp_zero_flow_identity():
     print("[warp-zero]", flush=True)
     dims = (16, 16, 16)
