@@ -591,7 +591,8 @@ def main():
     views_desc = str(Ps[0]) if len(set(Ps)) == 1 else f"{min(Ps)}-{max(Ps)} (seeded, total {sum(Ps)})"
     traffic = load_traffic(args.config, tile) if batched else None
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.config == "st256" else
+        "ST recon iters/sec at %dx%dx%d x %d frames; projector %% HBM peak" % (*cfg["vol"], cfg["F"]),
         "value": args.steps / elapsed,
         "unit": "iters/s",
         "n_gpus": 1,
