@@ -71,28 +71,40 @@ for _n, (_r, _a) in _sigs.items():
     _f.restype = _r
     _f.argtypes = _a
 
-ref = None
-if os.path.exists(REF_PATH):
-    ref = C.CDLL(REF_PATH)
-    for _n, (_r, _a) in {
-        "r_view_matrices": (C.c_int, [_G, _F, _F]),
-        "r_quat_y": (None, [C.c_float, _F]),
-        "r_quat_rotate_y": (None, [_F, C.c_float, _F]),
-        "r_matrix_inverse": (C.c_float, [_F, _F]),
-        "r_fp": (C.c_int, [_G, _F, _F, _F, _F, _F, _F, _F]),
-        "r_bp_ray_scatter": (C.c_int, [_G, _F, _F, _F, _F]),
-        "r_cubic_atxyz": (C.c_float, [_F, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
-        "r_of_diff": (C.c_int, [_I, _F, C.c_int, _F]),
-        "r_gradient_central": (C.c_int, [_I, _F, _F, _F, _F]),
-        "r_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
-        "r_resize": (C.c_int, [_I, C.c_int, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]),
-        "r_save_analyze": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F, C.c_int]),
-        "r_load_analyze": (C.c_int, [C.c_char_p, _I, _F, _F, C.c_size_t]),
-        "r_split_sizes": (C.c_int, [C.c_int, C.c_int, _I]),
-    }.items():
-        _f = getattr(ref, _n)
-        _f.restype = _r
-        _f.argtypes = _a
+# The reference-compiled checker is loaded on first use (O.ref), so test
+# processes that never call it -- every GPU test -- do not map it.
+HAVE_REF = os.path.exists(REF_PATH)
+_REF_SIGS = {
+    "r_view_matrices": (C.c_int, [_G, _F, _F]),
+    "r_quat_y": (None, [C.c_float, _F]),
+    "r_quat_rotate_y": (None, [_F, C.c_float, _F]),
+    "r_matrix_inverse": (C.c_float, [_F, _F]),
+    "r_fp": (C.c_int, [_G, _F, _F, _F, _F, _F, _F, _F]),
+    "r_bp_ray_scatter": (C.c_int, [_G, _F, _F, _F, _F]),
+    "r_cubic_atxyz": (C.c_float, [_F, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "r_of_diff": (C.c_int, [_I, _F, C.c_int, _F]),
+    "r_gradient_central": (C.c_int, [_I, _F, _F, _F, _F]),
+    "r_blur": (C.c_int, [_I, C.c_int, _F, C.c_float, _F]),
+    "r_resize": (C.c_int, [_I, C.c_int, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]),
+    "r_save_analyze": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F, C.c_int]),
+    "r_load_analyze": (C.c_int, [C.c_char_p, _I, _F, _F, C.c_size_t]),
+    "r_split_sizes": (C.c_int, [C.c_int, C.c_int, _I]),
+}
+_ref_lib = None
+
+
+def __getattr__(name):
+    global _ref_lib
+    if name != "ref":
+        raise AttributeError(name)
+    if _ref_lib is None and HAVE_REF:
+        lib = C.CDLL(REF_PATH)
+        for n, (r, a) in _REF_SIGS.items():
+            f = getattr(lib, n)
+            f.restype = r
+            f.argtypes = a
+        _ref_lib = lib
+    return _ref_lib
 
 
 def f32(a):

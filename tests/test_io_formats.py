@@ -26,7 +26,7 @@ import spacetimetomography_amd as stt
 PIL = pytest.importorskip("PIL")
 from PIL import Image, TiffImagePlugin  # noqa: E402
 
-needs_ref = pytest.mark.skipif(O.ref is None, reason="oracle/_ref not built (reference tree absent)")
+needs_ref = pytest.mark.skipif(not O.HAVE_REF, reason="oracle/_ref not built (reference tree absent)")
 
 
 def _rng(seed):

@@ -6,7 +6,7 @@ import pytest
 
 import _oracle as O
 
-needs_ref = pytest.mark.skipif(O.ref is None, reason="oracle/_ref not built (reference tree absent)")
+needs_ref = pytest.mark.skipif(not O.HAVE_REF, reason="oracle/_ref not built (reference tree absent)")
 
 
 def _vol(dims, seed):

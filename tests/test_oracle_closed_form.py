@@ -155,6 +155,90 @@ def test_warp_reproduces_quadratic_under_fractional_shift():
         np.testing.assert_allclose(out[sl], expect[sl], atol=2e-5)
 
 
+def _affine_warp_case(dims, seed):
+    """An affine velocity v(p) = A p + b (world = cell index + 0.5, voxel
+    units) and a linear image I(g) = c.g + d on the cell grid.  Trilinear and
+    clamped Catmull-Rom both reproduce linear data exactly and the 8-corner
+    clamp never binds on it, so warping.h:100-114's error-compensated trace
+    has a closed form: with tau = -dt,
+        p2 = p + tau v(p),  p0 = p2 - tau v(p2) = p - tau^2 A v(p),
+        x* = p2 + (p - p0)/2 = p + tau v(p) + tau^2 A v(p) / 2."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-0.06, 0.06, (3, 3))
+    b = rng.uniform(-1.5, 1.5, 3)
+    c = rng.uniform(-0.5, 0.5, 3)
+    d = 0.7
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in dims[::-1]), indexing="ij")
+    P = np.stack([x, y, z], -1) + 0.5                      # cell centres, world coordinates
+    v = P @ A.T + b
+    uvw = np.concatenate([v[..., k].reshape(-1) for k in range(3)]).astype(np.float32)
+    img = ((np.stack([x, y, z], -1) @ c) + d).astype(np.float32).reshape(-1)
+    return A, b, c, d, P, uvw, img
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+@pytest.mark.parametrize("dt", [-1.0, 1.0])
+def test_warp_trace_affine_flow_closed_form(interp, dt):
+    """A non-uniform (affine) flow makes v(p2) differ from v(p), so the
+    compensation term (p - p0)/2 is active.  Dropping it, flipping its sign
+    or sampling v(p) twice moves x* by tau^2 |A v|/2 or more, about 0.05
+    voxels here, against an fp32 tolerance of 2e-5."""
+    dims = (22, 20, 18)
+    A, b, c, d, P, uvw, img = _affine_warp_case(dims, seed=11)
+    tau = -dt
+    vp = P @ A.T + b
+    xstar = P + tau * vp + 0.5 * tau * tau * (vp @ A.T)
+    expect = (xstar - 0.5) @ c + d                          # I at grid coordinate x* - 0.5
+    p2 = P + tau * vp
+    # only voxels whose trace stays where no coordinate clamp ([1, dim-2]) binds
+    lim = np.array(dims, np.float64) - 2
+    ok = np.ones(P.shape[:3], bool)
+    for q in (P, p2, xstar):
+        g = q - 0.5
+        ok &= np.all((g >= 1.05) & (g <= lim - 0.05), -1)
+    assert ok.sum() > 1500
+    # the three plausible slips all move the result well beyond tolerance
+    wrong = {"dropped": P + tau * vp,
+             "flipped": P + tau * vp - 0.5 * tau * tau * (vp @ A.T)}
+    for name, q in wrong.items():
+        assert np.abs(((q - 0.5) @ c + d) - expect)[ok].max() > 50 * 2e-5, name
+    for f64mode in (False, True):
+        out = O.warp(dims, uvw, img, dt, interp, f64mode).reshape(dims[::-1])
+        err = np.abs(out - expect)[ok].max()
+        assert err <= (2e-5 if not f64mode else 2e-6), (f64mode, err)
+
+
+def test_bp_ray_apply_closed_form():
+    """x += contrib / wsum, then the reference's macro clamp max(min(G, x), 0)
+    (ST_Tomography.cpp:746-762, :167-173).  A voxel no ray reached has
+    contrib = wsum = 0: 0/0 is NaN, min(G, NaN) keeps NaN and max(NaN, 0)
+    gives 0, so it is zeroed (not left alone)."""
+    rng = np.random.default_rng(5)
+    n, G = 4000, 2.5
+    x = rng.uniform(0, G, n).astype(np.float32)
+    w = rng.uniform(0.05, 4.0, n).astype(np.float32)
+    c = (w * rng.normal(0, 1.0, n)).astype(np.float32)
+    w[::7] = 0.0
+    c[::7] = 0.0                       # unreached voxels
+    w[3::11] = 0.0
+    c[3::11] = np.where(np.arange(n)[3::11] % 2 == 0, 1.0, -1.0)  # +-x/0: +-inf
+    out = x.copy()
+    assert O.orc.o_bp_ray_apply_f32(n, O.f32(c), O.f32(w), G, O.f32(out)) == 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = c.astype(np.float64) / w
+    expect = np.clip(x + ratio, 0.0, G)
+    expect[np.isnan(ratio)] = 0.0
+    unreached = (w == 0) & (c == 0)
+    assert unreached.sum() > 400 and np.all(out[unreached] == 0.0)
+    assert np.all(out[(w == 0) & (c > 0)] == np.float32(G))
+    assert np.all(out[(w == 0) & (c < 0)] == 0.0)
+    np.testing.assert_allclose(out, expect, rtol=1e-6, atol=1e-6)
+    # no alpha: a unit ratio moves an interior voxel by exactly one
+    one = np.array([1.25], np.float32)
+    assert O.orc.o_bp_ray_apply_f32(1, O.f32(np.array([0.5], np.float32)), O.f32(np.array([0.5], np.float32)),
+                                    G, O.f32(one)) == 0 and one[0] == np.float32(2.25)
+
+
 def _pd_ctx(dims, params, F=2):
     g = stt.Geometry.make(dims, (16, 16))
     oc = O.OracleCtx(g, params, [1] * F)

@@ -74,7 +74,7 @@ def test_quaternions_bit_exact():
     assert np.array_equal(got, GOLD["quats"])
 
 
-@pytest.mark.skipif(O.ref is None, reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.skipif(not O.HAVE_REF, reason="oracle/_ref not built (reference tree absent)")
 @pytest.mark.parametrize("seed", range(8))
 def test_live_random_geometry_vs_reference_build(seed):
     rng = np.random.default_rng(1000 + seed)

@@ -20,7 +20,7 @@ def test_view_rounds_match_reference_composition_fixture():
     assert np.array_equal(q, g["round_quats"])
 
 
-@pytest.mark.skipif(O.ref is None, reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.skipif(not O.HAVE_REF, reason="oracle/_ref not built (reference tree absent)")
 def test_view_rounds_match_reference_build():
     rng = np.random.default_rng(21)
     starts = rng.uniform(0, 360, 5).astype(np.float32)
