@@ -57,6 +57,15 @@ size_t tail_pad(const int* d) {
     return (p + 63) / 64 * 64;
 }
 
+// Scale of the ray-BP weight sums: a view adds weights w <= 1, at most
+// max_samples per ray to one voxel, so a sum stays below D * max_samples <
+// 2^e; at 2^(62 - e) every int64 weight sum stays below 2^62.
+int raybp_weight_shift(const VolGeom& g) {
+    int e = 0;
+    (void)std::frexp((double)g.W * (double)g.H * (double)g.max_samples, &e);
+    return 62 - e;
+}
+
 // Owners released on every return path of the entry points that allocate.
 struct DevMem {
     void* p = nullptr;
@@ -402,7 +411,10 @@ struct stt_ctx {
     bool pd_ready = false;
 
     // device buffers
-    std::vector<float*> x, xT, xbar, xprev, ytv, ytp, ybc, flow, b, raylen_f, corr_f, contrib_f, wsum_f;
+    std::vector<float*> x, xT, xbar, xprev, ytv, ytp, ybc, flow, b, raylen_f, corr_f;
+    // ray BP: per-frame exact 64-bit fixed-point sums and their scale slots
+    std::vector<unsigned long long*> acc_c_f, acc_w_f;
+    int* acc_shift_dev = nullptr;
     float* raylen_shared = nullptr;
     float* zero_flow = nullptr;
     // End points of the PD warps' traces of each flow u_f (warping.h:100-114):
@@ -505,8 +517,10 @@ struct stt_ctx {
         it.b = b[f] + (size_t)view * D;
         it.raylen = shared_raylen ? raylen_shared : raylen_f[f];
         it.corr = corr_f[f];
-        it.contrib = contrib_f.empty() ? nullptr : contrib_f[f];
-        it.wsum = wsum_f.empty() ? nullptr : wsum_f[f];
+        it.contrib = nullptr;
+        it.acc_c = acc_c_f.empty() ? nullptr : acc_c_f[f];
+        it.acc_w = acc_w_f.empty() ? nullptr : acc_w_f[f];
+        it.acc_shift = acc_shift_dev ? acc_shift_dev + f : nullptr;
         it.xprev = xprev[f];
         it.xbar = xbar[f];
         it.frame = f;
@@ -587,7 +601,8 @@ struct stt_ctx {
 
     int launch_bp_for(const WorkItem* items_dev, int n, bool ray_bp) {
         if (ray_bp) {
-            HIPCHK(launch_fp(FP_MODE_SCATTER, items_dev, n, vg, 1.0f, stream));
+            HIPCHK(launch_raybp_prep(items_dev, n, vg, stream));
+            HIPCHK(launch_fp(FP_MODE_RAYBP, items_dev, n, vg, 1.0f, stream));
             HIPCHK(launch_bp_ray_apply(items_dev, n, vg, pc, stream));
         } else {
             HIPCHK(launch_bp_voxel(items_dev, n, vg, pc, stream));
@@ -657,7 +672,8 @@ struct stt_ctx {
                     HIPCHK(launch_transpose_xz(it, n, vg, sh));
                     HIPCHK(launch_fp(FP_MODE_SART, it, n, vg, pc.factor, sh));
                     if (ray_bp) {
-                        HIPCHK(launch_fp(FP_MODE_SCATTER, it, n, vg, 1.0f, sh));
+                        HIPCHK(launch_raybp_prep(it, n, vg, sh));
+                        HIPCHK(launch_fp(FP_MODE_RAYBP, it, n, vg, 1.0f, sh));
                         HIPCHK(launch_bp_ray_apply(it, n, vg, pc, sh));
                     } else {
                         HIPCHK(launch_bp_voxel(it, n, vg, pc, sh));
@@ -904,6 +920,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
         const float ulp = std::nextafter(cmax, INFINITY) - cmax;
         c->vg.box_eps = 1.0f / 32.0f + 0.5f * ulp * (float)c->vg.max_samples;
     }
+    c->vg.acc_wshift = raybp_weight_shift(c->vg);
     c->vg.vol_bytes = (unsigned)(c->Vpad * sizeof(float));
     c->vg.volT_bytes = (unsigned)(c->VpadT * sizeof(float));
     c->vg.det_bytes = (unsigned)((size_t)g->det[0] * g->det[1] * sizeof(float));
@@ -936,12 +953,13 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
         if ((rc = c->alloc(&c->corr_f[f], c->D))) return bail(rc);
     }
     if (p->bp == STT_BP_RAY) {
-        c->contrib_f.resize(F);
-        c->wsum_f.resize(F);
+        c->acc_c_f.resize(F);
+        c->acc_w_f.resize(F);
         for (int f = 0; f < F; ++f) {
-            if ((rc = c->alloc(&c->contrib_f[f], c->Vpad))) return bail(rc);
-            if ((rc = c->alloc(&c->wsum_f[f], c->Vpad))) return bail(rc);
+            if ((rc = c->alloc_t(&c->acc_c_f[f], c->Vpad))) return bail(rc);
+            if ((rc = c->alloc_t(&c->acc_w_f[f], c->Vpad))) return bail(rc);
         }
+        if ((rc = c->alloc_t(&c->acc_shift_dev, (size_t)F))) return bail(rc);
     }
     if ((rc = c->alloc(&c->raylen_shared, c->D))) return bail(rc);
     if ((rc = c->alloc(&c->zero_flow, 3 * c->V))) return bail(rc);
@@ -1173,23 +1191,37 @@ int stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr, fl
         return fail(STT_ERR_INVALID, "frame/view out of range");
     if (!corr) return fail(STT_ERR_INVALID, "null correction");
     if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
-    float* acc = nullptr;
-    HIPCHK(hipMallocAsync((void**)&acc, 2 * ctx->Vpad * sizeof(float), ctx->stream));
-    HIPCHK(hipMemsetAsync(acc, 0, 2 * ctx->Vpad * sizeof(float), ctx->stream));
+    const size_t Vp = ctx->Vpad, V = ctx->V;
+    DevMem acc;
+    HIPCHK(hipMalloc(&acc.p, 2 * Vp * sizeof(unsigned long long) + 256));
+    auto* ac = (unsigned long long*)acc.p;
+    auto* aw = ac + Vp;
+    int* shift = (int*)(aw + Vp);
+    HIPCHK(hipMemsetAsync(acc.p, 0, 2 * Vp * sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->corr_f[frame], corr, ctx->D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     WorkItem it = ctx->make_item(frame, view, true);
-    it.contrib = acc;
-    it.wsum = acc + ctx->Vpad;
-    it.raylen = nullptr;
+    it.acc_c = ac;
+    it.acc_w = aw;
+    it.acc_shift = shift;
+    it.raylen = nullptr;  // the caller's correction stands alone: no ray lengths to reset
     it.last = 0;
     int rc = ctx->upload_scratch({it});
     if (rc) return rc;
-    HIPCHK(launch_fp(FP_MODE_SCATTER, ctx->items_scratch, 1, ctx->vg, 1.0f, ctx->stream));
-    if (contrib) HIPCHK(hipMemcpyAsync(contrib, acc, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (wsum) HIPCHK(hipMemcpyAsync(wsum, acc + ctx->Vpad, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(launch_raybp_prep(ctx->items_scratch, 1, ctx->vg, ctx->stream));
+    HIPCHK(launch_fp(FP_MODE_RAYBP, ctx->items_scratch, 1, ctx->vg, 1.0f, ctx->stream));
+    std::vector<long long> hc, hw;
+    int hs = 0;
+    if (contrib || wsum) {
+        hc.resize(V);
+        hw.resize(V);
+        HIPCHK(hipMemcpyAsync(hc.data(), ac, V * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hw.data(), aw, V * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&hs, shift, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    }
     HIPCHK(launch_bp_ray_apply(ctx->items_scratch, 1, ctx->vg, ctx->pc, ctx->stream));
-    HIPCHK(hipFreeAsync(acc, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; contrib && k < V; ++k) contrib[k] = (float)std::ldexp((double)hc[k], -hs);
+    for (size_t k = 0; wsum && k < V; ++k) wsum[k] = (float)std::ldexp((double)hw[k], -ctx->vg.acc_wshift);
     return STT_OK;
 }
 
@@ -1208,7 +1240,6 @@ int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
     WorkItem it = ctx->make_item(frame, view, true);
     it.corr = dy;
     it.contrib = acc;
-    it.wsum = nullptr;
     it.raylen = nullptr;
     int rc = ctx->upload_scratch({it});
     if (rc) return rc;
@@ -1244,7 +1275,6 @@ int stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n) {
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(ctx->stream));  // the scratch slot is reused by the next view
     }
-    if (ray_bp) HIPCHK(hipMemsetAsync(ctx->raylen_shared, 0, ctx->D * sizeof(float), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }

@@ -41,6 +41,7 @@ struct VolGeom {
     int max_samples;     // hard cap of the sample loop (never reached by a valid ray)
     float box_eps;       // bound (cells) on how far a sample drifts off its exact line: max_samples
                          // rounded additions at coordinates of magnitude <= 2*max(dims), plus slack
+    int acc_wshift;      // ray-BP weight sums are int64 multiples of 2^-acc_wshift (see k_raybp_prep)
     unsigned vol_bytes;  // bytes of one padded volume allocation (buffer range)
     unsigned volT_bytes; // bytes of one padded transposed (z-fastest) volume
     unsigned det_bytes;  // bytes of one detector image
@@ -67,8 +68,10 @@ struct WorkItem {
     float* corr;         // correction image / ray-BP input (W*H)
     float* ax;           // forward projection output (stand-alone FP)
     float* weight;       // sample-count output (stand-alone FP)
-    float* contrib;      // ray-BP accumulators (padded V)
-    float* wsum;
+    float* contrib;      // fp32 scatter target of the projector's transpose (padded V)
+    unsigned long long* acc_c;  // ray-BP sums in 64-bit fixed point (two's complement, padded V):
+    unsigned long long* acc_w;  //   contributions w*corr at 2^-acc_shift[0], weights at 2^-acc_wshift
+    int* acc_shift;             //   this view's contribution scale, written by k_raybp_prep
     float* xprev;        // X_prev[f] (finalize step)
     float* xbar;         // X_bar[f] (finalize step)
     int frame;

@@ -71,6 +71,7 @@ __device__ __forceinline__ void produce_ray(const ViewRec& v, const VolGeom& g, 
 typedef __attribute__((address_space(1))) float gfloat;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) f32x4 gf32x4;
+typedef __attribute__((address_space(1))) unsigned long long gu64;
 __device__ __forceinline__ gfloat* gptr(float* p) { return (gfloat*)p; }
 __device__ __forceinline__ const gfloat* gptr(const float* p) { return (const gfloat*)p; }
 
@@ -170,39 +171,99 @@ __device__ __forceinline__ void fetch_taps_t(const VolView& v, float px, float p
     T.r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, off, v.sxynx4, 0));
 }
 
-// Scatter of the 8 trilinear weights (SingleScalarVolume.core.h:99-105, 128-143)
-// with hardware fp32 atomics.  Accumulators carry the same zero tail.
-__device__ __forceinline__ void scatter_tri(float* contrib, float* wsum, const VolGeom& g, float px, float py,
-                                            float pz, float val) {
+// Trilinear scatter of one sample (SingleScalarVolume.core.h:87-143): clamp
+// to [0, dims-1], the reference's 8 fp32 weights w[n], then per corner the
+// contribution w[n]*corr and the weight w[n] (UpdateContributionSum, :73-84).
+// Accumulators carry the volume's zero tail, so the +1 corners need no clamp
+// (their weights are 0 wherever the reference clamps them).
+struct TriCorners {
+    unsigned base;
+    unsigned off[8];
+    float w[8];
+};
+
+__device__ __forceinline__ void tri_corners(const VolGeom& g, float px, float py, float pz, TriCorners& c) {
     px = __builtin_amdgcn_fmed3f(px, 0.0f, g.bx);
     py = __builtin_amdgcn_fmed3f(py, 0.0f, g.by);
     pz = __builtin_amdgcn_fmed3f(pz, 0.0f, g.bz);
     const unsigned ix = (unsigned)px, iy = (unsigned)py, iz = (unsigned)pz;
     const float t0 = __builtin_amdgcn_fractf(px), t1 = __builtin_amdgcn_fractf(py), t2 = __builtin_amdgcn_fractf(pz);
     const float m0 = 1.0f - t0, m1 = 1.0f - t1, m2 = 1.0f - t2;
-    const float w[8] = {m0 * m1 * m2, t0 * m1 * m2, m0 * t1 * m2, t0 * t1 * m2,
-                        m0 * m1 * t2, t0 * m1 * t2, m0 * t1 * t2, t0 * t1 * t2};
+    c.w[0] = m0 * m1 * m2;
+    c.w[1] = t0 * m1 * m2;
+    c.w[2] = m0 * t1 * m2;
+    c.w[3] = t0 * t1 * m2;
+    c.w[4] = m0 * m1 * t2;
+    c.w[5] = t0 * m1 * t2;
+    c.w[6] = m0 * t1 * t2;
+    c.w[7] = t0 * t1 * t2;
     const unsigned nx = (unsigned)g.nx, sxy = (unsigned)(g.nx * g.ny);
-    const unsigned base = ix + __umul24(iy, nx) + __umul24(iz, sxy);
-    const unsigned off[8] = {0u, 1u, nx, nx + 1u, sxy, sxy + 1u, sxy + nx, sxy + nx + 1u};
-    gfloat* c = gptr(contrib) + base;
-#pragma unroll
-    for (int n = 0; n < 8; ++n) __builtin_amdgcn_global_atomic_fadd_f32(c + off[n], val * w[n]);
-    if (wsum) {
-        gfloat* s = gptr(wsum) + base;
-#pragma unroll
-        for (int n = 0; n < 8; ++n) __builtin_amdgcn_global_atomic_fadd_f32(s + off[n], w[n]);
-    }
+    c.base = ix + __umul24(iy, nx) + __umul24(iz, sxy);
+    c.off[0] = 0u;
+    c.off[1] = 1u;
+    c.off[2] = nx;
+    c.off[3] = nx + 1u;
+    c.off[4] = sxy;
+    c.off[5] = sxy + 1u;
+    c.off[6] = sxy + nx;
+    c.off[7] = sxy + nx + 1u;
 }
+
+// fp32 scatter with hardware atomics: the projector's transpose (A^T y).
+struct SinkF32 {
+    gfloat* acc;
+    float val;
+    __device__ __forceinline__ void operator()(const VolGeom& g, float px, float py, float pz) const {
+        TriCorners c;
+        tri_corners(g, px, py, pz, c);
+        gfloat* a = acc + c.base;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) __builtin_amdgcn_global_atomic_fadd_f32(a + c.off[n], val * c.w[n]);
+    }
+};
+
+// a * 2^s rounded to the nearest integer; exact wherever a's 24-bit mantissa
+// fits above 2^-s (|a| * 2^s < 2^62 by the choice of s)
+__device__ __forceinline__ unsigned long long fixed_of(float a, double scale) {
+    return (unsigned long long)__double2ll_rn((double)a * scale);
+}
+
+// Ray-BP scatter into exact 64-bit fixed-point sums.  Integer addition is
+// associative, so the sums -- and the volume update -- do not depend on the
+// order in which the lanes' atomics land: the result is the same on every
+// run.  Each addend is the reference's fp32 value (w[n]*corr, w[n]).
+struct SinkFixed {
+    unsigned long long* acc_c;
+    unsigned long long* acc_w;
+    float val;
+    double cscale, wscale;
+    __device__ __forceinline__ void operator()(const VolGeom& g, float px, float py, float pz) const {
+        TriCorners c;
+        tri_corners(g, px, py, pz, c);
+        gu64* ac = (gu64*)acc_c + c.base;
+        gu64* aw = (gu64*)acc_w + c.base;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            __hip_atomic_fetch_add(ac + c.off[n], fixed_of(c.w[n] * val, cscale), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(aw + c.off[n], fixed_of(c.w[n], wscale), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+};
 
 // CastingRay (RayCaster.core.h:42-187) with one block = the whole volume
 // (RayCasterPrepare :199-201).  Samples: the entry point once more than the
 // grid (:60-61), then t_k = tnear + k*sd below the next block boundary; the only
 // interior boundaries are the planes at voxel coordinate 0.5 crossed in the
 // negative direction, where the grid restarts at ceil((t_b - tnear)/sd).
-template <bool SCATTER>
-__device__ __forceinline__ float cast_ray(const VolView& vol, float* contrib, float* wsum, float sval,
-                                          const VolGeom& g, const Ray& r, int& nsamp) {
+struct SinkNone {
+    __device__ __forceinline__ void operator()(const VolGeom&, float, float, float) const {}
+};
+
+template <bool SCATTER, class Sink>
+__device__ __forceinline__ float cast_ray(const VolView& vol, const Sink& sink, const VolGeom& g, const Ray& r,
+                                          int& nsamp) {
     const float sd = g.sd;
     const int dimv[3] = {g.nx, g.ny, g.nz};
     float pos1[3];
@@ -216,7 +277,7 @@ __device__ __forceinline__ float cast_ray(const VolView& vol, float* contrib, fl
     for (int c = 0; c < 3; ++c) step[c] = r.d[c] * sd;
     float density = 0.0f;
     int n = 1;
-    if (SCATTER) scatter_tri(contrib, wsum, g, pos1[0], pos1[1], pos1[2], sval);
+    if (SCATTER) sink(g, pos1[0], pos1[1], pos1[2]);
     else density = sample_tri(vol, pos1[0], pos1[1], pos1[2]) * sd;
     float tMax[3], tDelta[3];
     {
@@ -252,7 +313,7 @@ __device__ __forceinline__ float cast_ray(const VolView& vol, float* contrib, fl
         }
         if (SCATTER) {
             while (t < tb && n < g.max_samples) {
-                scatter_tri(contrib, wsum, g, px, py, pz, sval);
+                sink(g, px, py, pz);
                 ++n;
                 t += sd;
                 px += step[0];
@@ -318,19 +379,29 @@ __global__ __launch_bounds__(TX* TY) void k_fp(const WorkItem* __restrict__ item
     produce_ray(v, g, (float)i, (float)j, r);
     const bool hit = r.tnear < r.tfar;
     if (MODE == FP_MODE_SCATTER) {
-        // Ray BP / adjoint: the BP-mode pass of GenerateImage_MT.
+        // the projector's transpose: factor*y[k] scattered with the weights
         if (hit) {
             int ns;
-            const float c = gptr(it.corr)[k] * factor;
-            VolView none{};
-            cast_ray<true>(none, it.contrib, it.wsum, c, g, r, ns);
-            if (it.raylen) gptr(it.raylen)[k] = r.tfar - r.tnear;
+            const SinkF32 sink{gptr(it.contrib), gptr(it.corr)[k] * factor};
+            cast_ray<true>(VolView{}, sink, g, r, ns);
+        }
+        return;
+    }
+    if (MODE == FP_MODE_RAYBP) {
+        // Ray BP: the BP-mode pass of GenerateImage_MT over the same samples
+        // (ST_Tomography.cpp:733-739).  Its ray lengths equal the SART pass's
+        // and are cleared right after (Initial(), :763), so none is written.
+        if (hit) {
+            int ns;
+            const SinkFixed sink{it.acc_c, it.acc_w, gptr(it.corr)[k], ldexp(1.0, *it.acc_shift),
+                                 ldexp(1.0, g.acc_wshift)};
+            cast_ray<true>(VolView{}, sink, g, r, ns);
         }
         return;
     }
     float value = 0.0f;
     int ns = 0;
-    if (hit) value = cast_ray<false>(vol_view(it.vol, g), nullptr, nullptr, 0.0f, g, r, ns);
+    if (hit) value = cast_ray<false>(vol_view(it.vol, g), SinkNone{}, g, r, ns);
     if (MODE == FP_MODE_FP) {
         if (hit && it.raylen) gptr(it.raylen)[k] = r.tfar - r.tnear;
         if (it.ax) gptr(it.ax)[k] = value;
@@ -1276,8 +1347,50 @@ __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ i
         if (i < nz_here) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, xs[i]), rv, vo, (z0 + i) * plane4, 0);
 }
 
-// Ray-BP normalisation (ST_Tomography.cpp:746-762): x += contrib/wsum with the
-// macro clamp (0/0 -> NaN -> 0); clears the accumulators (Initial(), :763).
+// Ray-BP set-up of one view, one block per work item, after the SART pass
+// has written the correction: the scale of the fixed-point contribution sums
+// and the reference's Initial() of the ray lengths.
+//   Scale: every addend w*corr has |w| <= 1 and one ray adds at most
+//   max_samples of them to a voxel, so a view's sums stay below
+//   D * max_samples * max|corr|.  acc_shift = 62 - e with that bound < 2^e
+//   keeps every sum below 2^62 in the int64 accumulators (the weights use
+//   the same bound with max|corr| = 1: VolGeom::acc_wshift).
+//   Ray lengths: TypicalRayProducer::Initial() (TypicalRayProducer.cpp:78-96,
+//   called at ST_Tomography.cpp:763) zeroes them after every ray-BP view, so
+//   the next view's missed rays read 0 rather than a stale length.
+__global__ __launch_bounds__(1024) void k_raybp_prep(const WorkItem* __restrict__ items, VolGeom g) {
+    __shared__ float red[16];
+    const WorkItem& it = items[blockIdx.x];
+    const int D = g.W * g.H;
+    const gfloat* corr = gptr(it.corr);
+    gfloat* rl = it.raylen ? gptr(it.raylen) : nullptr;
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < D; k += 1024) {
+        m = fmaxf(m, fabsf(corr[k]));
+        if (rl) rl[k] = 0.0f;
+    }
+    m = wave_max_f32(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = red[0];
+        for (int w = 1; w < 16; ++w) a = fmaxf(a, red[w]);
+        int e = 0;
+        const double bound = (double)a * (double)D * (double)g.max_samples;
+        int shift = 0;
+        if (bound > 0.0 && bound < 1e300) {
+            (void)frexp(bound, &e);  // bound < 2^e
+            shift = 62 - e;
+        } else if (bound == 0.0) {
+            shift = 0;  // every addend is 0
+        }
+        *it.acc_shift = shift;
+    }
+}
+
+// Ray-BP normalisation (ST_Tomography.cpp:746-762): x += contrib/wsum in
+// double (no alpha) with the macro clamp, so a voxel no ray reached (0/0 ->
+// NaN) becomes 0; then the accumulators are cleared (Initial(), :763).
 __global__ __launch_bounds__(256) void k_bp_ray_apply(const WorkItem* __restrict__ items, VolGeom g, PdConst pc) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1285,14 +1398,16 @@ __global__ __launch_bounds__(256) void k_bp_ray_apply(const WorkItem* __restrict
     const WorkItem& it = items[blockIdx.z / g.nz];
     if (x >= g.nx || y >= g.ny) return;
     const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    gfloat* contrib = gptr(it.contrib);
-    gfloat* wsum = gptr(it.wsum);
+    gu64* ac = (gu64*)it.acc_c;
+    gu64* aw = (gu64*)it.acc_w;
     gfloat* vol = gptr(it.vol);
-    const float m = (float)((double)contrib[q] / (double)wsum[q]);
+    const double contrib = ldexp((double)(long long)ac[q], -*it.acc_shift);
+    const double wsum = ldexp((double)(long long)aw[q], -g.acc_wshift);
+    const float m = (float)(contrib / wsum);
     float xv = vol[q] + m;
     xv = tmax(tmin(pc.gmax, xv), 0.0f);  // ternaries: NaN from 0/0 becomes 0 as in the reference
-    contrib[q] = 0.0f;
-    wsum[q] = 0.0f;
+    ac[q] = 0ull;
+    aw[q] = 0ull;
     if (it.last) xv = finalize_voxel(it, g, pc, x, y, z, q, xv);
     vol[q] = xv;
 }
@@ -2220,6 +2335,7 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
             return g_fp_tile == 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
                                   : launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
+        case FP_MODE_RAYBP: hipLaunchKernelGGL((k_fp<FP_MODE_RAYBP, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2236,6 +2352,11 @@ hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, 
     const dim3 grid((g.nx + kBpTileX - 1) / kBpTileX, (g.ny + kBpTileY - 1) / kBpTileY,
                     ((g.nz + kBpVoxZ - 1) / kBpVoxZ) * nitems);
     hipLaunchKernelGGL(k_bp_voxel, grid, dim3(256), 0, s, items, g, pc);
+    return hipGetLastError();
+}
+
+hipError_t launch_raybp_prep(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_raybp_prep, dim3(nitems), dim3(1024), 0, s, items, g);
     return hipGetLastError();
 }
 

@@ -6,12 +6,14 @@
 
 namespace stt {
 
-enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2 };
+enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2, FP_MODE_RAYBP = 3 };
 
 // Projector over nitems work items (one per frame), on stream s.
 //   FP_MODE_FP      : ax, raylen (hit rays), weight
 //   FP_MODE_SART    : corr = (F b - F Ax) / (F len + 1), raylen updated
-//   FP_MODE_SCATTER : contrib += factor*corr*w, wsum += w (wsum may be null)
+//   FP_MODE_SCATTER : contrib += factor*corr*w (fp32 atomics; the projector's transpose)
+//   FP_MODE_RAYBP   : acc_c += corr*w, acc_w += w in exact 64-bit fixed point
+//                     (ray BP; launch_raybp_prep first sets the scale)
 // Projector shape: 0 = slab projector (LDS-staged boxes, default); 64, 32,
 // 16 or 8 = level-synchronous projector with that many rays per wave row.
 void set_fp_tile(int tx);
@@ -22,6 +24,9 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
 // (level-synchronous projector only; a no-op for the slab projector).
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s);
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
+// Ray BP, per view: scale of the fixed-point sums and the reset of the ray
+// lengths (after the SART pass), then scatter (FP_MODE_RAYBP), then apply.
+hipError_t launch_raybp_prep(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s);
 hipError_t launch_bp_ray_apply(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s);
 // End points of the PD traces of one flow (step +1 into tb, -1 into tf).
 hipError_t launch_trace(const float* uvw, float* tb, float* tf, const VolGeom& g, int interp, hipStream_t s);

@@ -1,0 +1,177 @@
+"""GPU parity of the ray-driven backprojector inside SART sweeps and ST
+iterations, and of the reference's region of interest, against the oracle.
+
+Ray BP (ST_Tomography.cpp:733-765): per view a BP-mode pass scatters
+w * corr and w into two sums, the volume takes x += contrib / wsum (no
+alpha; a voxel no ray reached gets 0/0 and is zeroed by the clamp), and
+TypicalRayProducer::Initial() (TypicalRayProducer.cpp:78-96) clears the sums
+and the ray lengths.  The detectors here are smaller than the volume's
+shadow in one direction and wider in the other, so every view leaves
+voxels unreached and has rays that miss the volume; the
+measured projections carry noise on every pixel, so a missed ray's
+correction b / (raylen + 1) depends on the reset ray length.
+
+Reference ROI (ST_Tomography.cpp:114, 303-329, 793-804): the rose profile's
+cut[] = {8,8,8,45,8,8,30,20} and the j > dims[1]-7 band at 64^3, where the
+chamfered x-z corners, the band and the PD loops' narrow j range
+(j in [8, 19]) are all active.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import spacetimetomography_amd as stt
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(vol, det, P, params, ds=0.8, sid=60.0, sdd=100.0, seed=31, noise=0.05, flows=True):
+    g = stt.Geometry.make(vol, det, (1, 1, 1), ds, sid, sdd)
+    ctx = stt.Context(g, params, list(P))
+    oc = O.OracleCtx(g, params, list(P))
+    rng = np.random.default_rng(seed)
+    F = len(P)
+    V = int(np.prod(vol))
+    off = 0
+    for f in range(F):
+        ang = list(stt.synth_angles(1, P[f], offset=off))
+        off += P[f]
+        ctx.set_views(f, ang)
+        oc.set_views(f, ang)
+        truth = stt.synth_phantom_frame(vol, f, n_ellipsoids=6, seed=1, deform_seed=2, amp=2.0)
+        b = np.concatenate([O.fp(g, O.quat_y(a), truth)[0] for a in ang])
+        b = (b + noise * rng.standard_normal(b.size)).astype(np.float32)
+        ctx.set_projections(f, b)
+        oc.set_projections(f, b)
+    for f in range(F + 1):
+        x = (rng.random(V, dtype=np.float32) * 0.2).astype(np.float32)
+        ctx.set_volume(f, x)
+        oc.set_volume(f, x)
+    if flows:
+        for f in range(F - 1):
+            u = stt.synth_flow(vol, f, deform_seed=2, amp=2.0)
+            ctx.set_flow(f, u)
+            oc.set_flow(f, u)
+    return g, ctx, oc
+
+
+def _ray_params():
+    p = stt.params_default()
+    p.bp = stt.STT_BP_RAY
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01
+    return p
+
+
+@pytest.mark.parametrize("shape", [0, 16])
+def test_ray_bp_sweep_matches_oracle(shape):
+    vol, det = (28, 28, 28), (96, 24)
+    print(f"[raybp-sweep] {vol} det {det} at 0.8 mm, 10 views, projector shape {shape}", flush=True)
+    g, ctx, oc = _pair(vol, det, (10,), _ray_params(), flows=False)
+    order = np.random.default_rng(4).permutation(10).astype(np.int32)
+    try:
+        ctx.set_projector_tile(shape)
+        ctx.sart_sweep(0, order)
+    finally:
+        ctx.set_projector_tile(0)
+    oc.sart(0, order)
+    got, ref = ctx.get_volume(0), oc.get_volume(0)
+    err = rel_l2(got, ref)
+    zeros = int(np.count_nonzero(ref == 0))
+    mismatch = int(np.count_nonzero((got == 0) != (ref == 0)))
+    print(f"  relL2 vs oracle {err:.2e}; zero voxels {zeros} of {ref.size}, zero-set mismatches {mismatch}")
+    assert zeros > ref.size // 20  # unreached voxels were zeroed (0/0 -> 0)
+    assert mismatch <= ref.size // 1000
+    assert err <= 1e-4, err
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "graph"])
+def test_ray_bp_iteration_matches_oracle(mode):
+    vol, det, P = (24, 24, 24), (84, 20), (5, 6, 4)
+    print(f"[raybp-iter] mode={mode} F=3 {vol} det {det} P={P}", flush=True)
+    g, ctx, oc = _pair(vol, det, P, _ray_params())
+    batched = mode != "plain"
+    ctx.iterate(2, batched=batched, use_graph=(mode == "graph"))
+    # ray BP resets the ray lengths after every view, so the shared (plain)
+    # and per-frame (batched) stale buffers give the same result
+    oc.per_frame_raylen(not batched)
+    oc.iterate(2)
+    for f in range(3):
+        got, ref = ctx.get_volume(f), oc.get_volume(f)
+        err = rel_l2(got, ref)
+        print(f"  frame {f}: relL2 vs oracle {err:.2e}", flush=True)
+        assert err <= 1e-4, (f, err)
+        assert rel_l2(ctx.get_buffer(stt.STT_BUF_XBAR, f), oc.get_buffer(stt.STT_BUF_XBAR, f)) <= 1e-4
+    ctx.close()
+
+
+def test_ray_bp_is_deterministic():
+    """The ray-BP sums are exact 64-bit fixed point: two runs of the same
+    sweep give bit-identical volumes, and the same scatter twice gives
+    bit-identical sums."""
+    vol, det = (48, 48, 48), (64, 64)
+    print(f"[raybp-determinism] {vol} det {det}, 12 views, two contexts", flush=True)
+    outs = []
+    for _ in range(2):
+        g, ctx, oc = _pair(vol, det, (12,), _ray_params(), ds=1.0, sid=90.0, sdd=150.0, flows=False)
+        del oc
+        ctx.sart_sweep(0, None)
+        outs.append(ctx.get_volume(0))
+        corr = np.random.default_rng(2).standard_normal(det[0] * det[1]).astype(np.float32)
+        c1, w1 = ctx.backproject_ray(0, 3, corr)
+        c2, w2 = ctx.backproject_ray(0, 3, corr)
+        assert np.array_equal(c1, c2) and np.array_equal(w1, w2)
+        ctx.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def _ref_params(bp):
+    p = stt.params_reference()
+    p.bp = bp
+    p.k_tp, p.e_bc = 0.1, 0.05
+    return p
+
+
+@pytest.mark.parametrize("bp", [stt.STT_BP_VOXEL, stt.STT_BP_RAY])
+def test_reference_roi_sweep_matches_oracle(bp):
+    vol, det = (64, 64, 64), (160, 96)
+    print(f"[roi-sweep] reference profile bp={bp} {vol} det {det}, 12 views", flush=True)
+    p = _ref_params(bp)
+    g, ctx, oc = _pair(vol, det, (12,), p, ds=1.0, sid=120.0, sdd=200.0, flows=False)
+    order = np.random.default_rng(5).permutation(12).astype(np.int32)
+    ctx.sart_sweep(0, order)
+    oc.sart(0, order)
+    got, ref = ctx.get_volume(0), oc.get_volume(0)
+    roi = O.roi_mask(vol, list(p.roi_cut), p.roi_jtop)
+    out = roi == 0
+    print(f"  ROI keeps {int(roi.sum())} of {roi.size} voxels; relL2 inside {rel_l2(got[~out], ref[~out]):.2e}",
+          flush=True)
+    assert out.sum() > roi.size // 2 and (~out).sum() > roi.size // 10
+    assert np.all(got[out] == 0) and np.all(ref[out] == 0)
+    # the chamfered x-z corners and the top band are cut, the centre is kept
+    r = roi.reshape(vol[::-1])
+    assert r[32, 32, 32] == 1 and r[10, 32, 10] == 0 and r[32, 60, 32] == 0
+    assert rel_l2(got[~out], ref[~out]) <= 1e-4
+    ctx.close()
+
+
+@pytest.mark.parametrize("bp", [stt.STT_BP_VOXEL, stt.STT_BP_RAY])
+def test_reference_roi_iteration_matches_oracle(bp):
+    vol, det, P = (64, 64, 64), (160, 96), (6, 5)
+    print(f"[roi-iter] reference profile bp={bp} F=2 {vol} det {det}, batched graph", flush=True)
+    p = _ref_params(bp)
+    g, ctx, oc = _pair(vol, det, P, p, ds=1.0, sid=120.0, sdd=200.0)
+    ctx.iterate(1, batched=True, use_graph=True)
+    oc.per_frame_raylen(True)
+    oc.iterate(1)
+    roi = O.roi_mask(vol, list(p.roi_cut), p.roi_jtop) != 0
+    for f in range(2):
+        got, ref = ctx.get_volume(f), oc.get_volume(f)
+        assert np.all(got[~roi] == 0)
+        err = rel_l2(got, ref)
+        xb = rel_l2(ctx.get_buffer(stt.STT_BUF_XBAR, f), oc.get_buffer(stt.STT_BUF_XBAR, f))
+        ytv = rel_l2(ctx.get_buffer(stt.STT_BUF_YTV, f), oc.get_buffer(stt.STT_BUF_YTV, f))
+        print(f"  frame {f}: x {err:.2e}, X_bar {xb:.2e}, Y_tv {ytv:.2e}", flush=True)
+        assert err <= 1e-4 and xb <= 1e-4 and ytv <= 1e-4, (f, err, xb, ytv)
+    ctx.close()
