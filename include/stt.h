@@ -253,6 +253,18 @@ int  stt_synchronize(stt_ctx* ctx);
  * r2 = sum over frames f and views k of ||b_fk - A_fk x_f||^2 (one forward
  * projection per view into scratch; no solver state changes). */
 int  stt_residual_norm(stt_ctx* ctx, double* r2);
+/* Same *r2 as stt_residual_norm, bit for bit, without allocating: no
+ * hipMalloc, hipFree or other allocating call.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid it).  Its
+ * scratch -- one work item, one view's line integrals and ray lengths, the
+ * FP64 partials and running sums -- is a file-scope device array of about
+ * 8 MiB that each device keeps until the process ends, so detectors are
+ * limited to 2^20 pixels (STT_ERR_INVALID beyond).  The context's device
+ * must be current (STT_ERR_STATE otherwise).  Every call on a device shares
+ * that device's array: calls that overlap on one device race (contexts on
+ * different streams included), so serialise them. */
+int  stt_residual_norm_noalloc(stt_ctx* ctx, double* r2);
 /* Power iteration (iters steps from a fixed pseudo-random start) for the
  * norm of the PD coupling operator K over the context's F frames:
  * K v = (forward differences of each frame along x, y, z on the dual's

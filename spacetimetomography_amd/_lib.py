@@ -119,6 +119,7 @@ _SIGS = {
     "stt_set_projector_tile": (C.c_int, [_P, C.c_int]),
     "stt_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "stt_residual_norm": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "stt_residual_norm_noalloc": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "stt_power_iteration": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "stt_synth_phantom": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _F]),
     "stt_synth_phantom_frame": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,

@@ -1704,8 +1704,8 @@ int stt_residual_norm(stt_ctx* ctx, double* r2) {
         HIPCHK(launch_fp(FP_MODE_FP, dev_items + off[k], cnt[k], ctx->vg, 1.0f, ctx->stream));
         for (int n = 0; n < cnt[k]; ++n) {
             const WorkItem& it = items[off[k] + n];
-            HIPCHK(launch_residual_partial(it.b, it.ax, (int)D, partial + (size_t)(off[k] + n) * kReductionBlocks,
-                                           ctx->stream));
+            // block-major: block t of every view at partial[t * nitems + view]
+            HIPCHK(launch_residual_partial(it.b, it.ax, (int)D, partial + (off[k] + n), nitems, ctx->stream));
         }
     }
     double* out = partial + (size_t)nitems * kReductionBlocks;
@@ -1714,6 +1714,49 @@ int stt_residual_norm(stt_ctx* ctx, double* r2) {
     HIPCHK(hipFreeAsync(ax, ctx->stream));
     HIPCHK(hipFreeAsync(partial, ctx->stream));
     HIPCHK(hipFreeAsync(dev_items, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+int stt_residual_norm_noalloc(stt_ctx* ctx, double* r2) {
+    CTX_CHECK(ctx);
+    if (!r2) return fail(STT_ERR_INVALID, "null argument");
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    if (ctx->D > kNoAllocMaxDet)
+        return fail(STT_ERR_INVALID, "stt_residual_norm_noalloc: detector of " + std::to_string(ctx->D) +
+                                         " pixels exceeds its scratch limit of " + std::to_string(kNoAllocMaxDet));
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != ctx->device)
+        return fail(STT_ERR_STATE, "stt_residual_norm_noalloc: make the context's device " +
+                                       std::to_string(ctx->device) + " current");
+    char* base = nullptr;
+    HIPCHK((hipError_t)noalloc_scratch(&base));
+    auto* item = (WorkItem*)(base + kNoAllocItemOff);
+    auto* part = (double*)(base + kNoAllocPartOff);
+    auto* acc = (double*)(base + kNoAllocAccOff);
+    auto* out = (double*)(base + kNoAllocOutOff);
+    auto* ax = (float*)(base + kNoAllocAxOff);
+    auto* rl = (float*)(base + kNoAllocRlOff);
+    const size_t D = ctx->D;
+    // The views in stt_residual_norm's order (view index k, then frame), one
+    // at a time: projection into scratch, its partials, then each block's
+    // running sum -- the additions k_sum_partials makes over the block-major
+    // partials of all views, in the same order, so *r2 is the same double.
+    HIPCHK(launch_zero_f64(acc, kReductionBlocks, ctx->stream));
+    for (int k = 0; k < ctx->maxP; ++k)
+        for (int f = 0; f < ctx->F; ++f) {
+            if (k >= ctx->P[f]) continue;
+            const float* b = ctx->b[f] + (size_t)k * D;
+            HIPCHK(launch_item_make(item, ctx->views_dev + ctx->view_off[f] + k, ctx->x[f], ctx->xT[f], b, ax, rl,
+                                    ctx->stream));
+            HIPCHK(launch_fp(FP_MODE_FP, item, 1, ctx->vg, 1.0f, ctx->stream));
+            HIPCHK(launch_residual_partial(b, ax, (int)D, part, 1, ctx->stream));
+            HIPCHK(launch_add_partials(acc, part, kReductionBlocks, ctx->stream));
+        }
+    HIPCHK(launch_sum_partials(acc, kReductionBlocks, out, ctx->stream));
+    HIPCHK(hipMemcpyAsync(r2, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }

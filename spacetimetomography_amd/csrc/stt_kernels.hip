@@ -17,6 +17,8 @@
 // which is the batched form; the per-frame form launches one frame.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "stt_device.hpp"
 #include "stt_kernels.hpp"
 
@@ -1733,9 +1735,12 @@ __device__ __forceinline__ double block_sum_f64(double v, double* red) {
     return t;  // valid in thread 0
 }
 
-// partial[blockIdx.x] = sum over k of (b[k] - ax[k])^2 (ax = 0 for a missed ray).
+// partial[blockIdx.x * stride] = sum over k of (b[k] - ax[k])^2 (ax = 0 for a
+// missed ray).  With one view per slot and stride = number of views, block
+// t's partials of all views are contiguous, so k_sum_partials adds each
+// block's partials over the views in view order.
 __global__ __launch_bounds__(256) void k_residual_partial(const float* __restrict__ b, const float* __restrict__ ax,
-                                                          int n, double* __restrict__ partial) {
+                                                          int n, double* __restrict__ partial, int stride) {
     __shared__ double red[4];
     double acc = 0.0;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
@@ -1743,7 +1748,7 @@ __global__ __launch_bounds__(256) void k_residual_partial(const float* __restric
         acc = fma(r, r, acc);
     }
     const double t = block_sum_f64(acc, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * stride] = t;
 }
 
 // partial[blockIdx.x] = sum of a[k]*c[k] (a == c: squared norm).
@@ -1766,6 +1771,38 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
     for (int k = threadIdx.x * per; k < min(n, (int)(threadIdx.x + 1) * per); ++k) acc += partial[k];
     const double t = block_sum_f64(acc, red);
     if (threadIdx.x == 0) out[0] = t;
+}
+
+// ---- allocation-free residual norm -----------------------------------------
+// File-scope scratch of stt_residual_norm_noalloc, one copy per device.  In
+// host code the name `scratch` is only a host-side stand-in: its device
+// address comes from hipGetSymbolAddress (noalloc_scratch), and only that
+// address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// One work item of the projector, assembled on the device from the
+// context's device pointers (no host-to-device copy of a host temporary).
+__global__ void k_item_make(WorkItem* __restrict__ dst, const ViewRec* view, float* vol, float* volT,
+                            const float* b, float* ax, float* raylen) {
+    WorkItem it{};
+    it.view = view;
+    it.vol = vol;
+    it.volT = volT;
+    it.b = b;
+    it.ax = ax;
+    it.raylen = raylen;
+    *dst = it;
+}
+
+__global__ __launch_bounds__(256) void k_zero_f64(double* __restrict__ p, int n) {
+    for (int k = threadIdx.x; k < n; k += 256) p[k] = 0.0;
+}
+
+// acc[t] += part[t]: block t's running sum over the views, added in view
+// order exactly as k_sum_partials adds a block-major partial array.
+__global__ __launch_bounds__(256) void k_add_partials(double* __restrict__ acc, const double* __restrict__ part,
+                                                      int n) {
+    for (int k = threadIdx.x; k < n; k += 256) acc[k] += part[k];
 }
 
 // x *= s (s = 1/||x|| read from device memory as a double).
@@ -2395,8 +2432,9 @@ hipError_t launch_warp_vjp(const float* uvw, const float* src, const float* gout
     return hipGetLastError();
 }
 
-hipError_t launch_residual_partial(const float* b, const float* ax, int n, double* partial, hipStream_t s) {
-    hipLaunchKernelGGL(k_residual_partial, dim3(kReductionBlocks), dim3(256), 0, s, b, ax, n, partial);
+hipError_t launch_residual_partial(const float* b, const float* ax, int n, double* partial, int stride,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(k_residual_partial, dim3(kReductionBlocks), dim3(256), 0, s, b, ax, n, partial, stride);
     return hipGetLastError();
 }
 
@@ -2407,6 +2445,42 @@ hipError_t launch_dot_partial(const float* a, const float* c, size_t n, double* 
 
 hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, n, out);
+    return hipGetLastError();
+}
+
+// Device address of `scratch` on the current device: looked up once per
+// device (hipGetSymbolAddress, with that device current) and kept.
+int noalloc_scratch(char** ptr) {
+    static std::mutex mu;
+    static char* addr[kNoAllocMaxDevices] = {};
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        addr[dev] = (char*)p;
+    }
+    *ptr = addr[dev];
+    return (int)hipSuccess;
+}
+
+hipError_t launch_item_make(WorkItem* dst, const ViewRec* view, float* vol, float* volT, const float* b, float* ax,
+                            float* raylen, hipStream_t s) {
+    hipLaunchKernelGGL(k_item_make, dim3(1), dim3(1), 0, s, dst, view, vol, volT, b, ax, raylen);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f64(double* p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_partials(double* acc, const double* part, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(256), 0, s, acc, part, n);
     return hipGetLastError();
 }
 

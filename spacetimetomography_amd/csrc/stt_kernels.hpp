@@ -41,10 +41,35 @@ hipError_t launch_warp_vjp(const float* uvw, const float* src, const float* gout
                            const VolGeom& g, float dt, hipStream_t s);
 // FP64 reductions: kReductionBlocks partials per call, then one ordered final sum.
 constexpr int kReductionBlocks = 256;
-hipError_t launch_residual_partial(const float* b, const float* ax, int n, double* partial, hipStream_t s);
+// partial[t * stride], t < kReductionBlocks
+hipError_t launch_residual_partial(const float* b, const float* ax, int n, double* partial, int stride,
+                                   hipStream_t s);
 hipError_t launch_dot_partial(const float* a, const float* c, size_t n, double* partial, hipStream_t s);
 hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
 hipError_t launch_scale_by_inv_sqrt(float* x, size_t n, const double* nrm2, hipStream_t s);
+
+// Allocation-free residual norm (stt_residual_norm_noalloc).  A file-scope
+// device array of kNoAllocScratchBytes holds, per call: one work item, the
+// current view's line integrals and ray lengths (detectors up to
+// kNoAllocMaxDet pixels), its kReductionBlocks FP64 partials, the blocks'
+// running sums and the result.  noalloc_scratch() returns the array's
+// address on the current device (hipError_t as int).
+constexpr size_t kNoAllocMaxDet = size_t(1) << 20;
+constexpr size_t kNoAllocItemOff = 0;                                    // one WorkItem (256 B reserved)
+constexpr size_t kNoAllocPartOff = 256;                                  // kReductionBlocks doubles
+constexpr size_t kNoAllocAccOff = kNoAllocPartOff + 8 * kReductionBlocks;  // kReductionBlocks doubles
+constexpr size_t kNoAllocOutOff = kNoAllocAccOff + 8 * kReductionBlocks;   // 1 double
+constexpr size_t kNoAllocAxOff = 8192;                                   // kNoAllocMaxDet floats
+constexpr size_t kNoAllocRlOff = kNoAllocAxOff + 4 * kNoAllocMaxDet;      // kNoAllocMaxDet floats
+constexpr size_t kNoAllocScratchBytes = kNoAllocRlOff + 4 * kNoAllocMaxDet;
+constexpr int kNoAllocMaxDevices = 64;
+static_assert(sizeof(WorkItem) <= kNoAllocPartOff, "work item slot too small");
+static_assert(kNoAllocOutOff + 8 <= kNoAllocAxOff, "scratch layout overlaps");
+int noalloc_scratch(char** ptr);
+hipError_t launch_item_make(WorkItem* dst, const ViewRec* view, float* vol, float* volT, const float* b, float* ax,
+                            float* raylen, hipStream_t s);
+hipError_t launch_zero_f64(double* p, int n, hipStream_t s);
+hipError_t launch_add_partials(double* acc, const double* part, int n, hipStream_t s);
 hipError_t launch_fill_hash(float* x, size_t n, unsigned seed, hipStream_t s);
 // Power-iteration operator K (masked spatial forward differences + time difference) and K^T.
 hipError_t launch_pi_apply_k(const float* v, float* y, int F, const VolGeom& g, const PdConst& pc, hipStream_t s);
