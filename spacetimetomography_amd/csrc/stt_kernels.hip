@@ -701,7 +701,7 @@ constexpr float kSlabShear = 0.7f;  // slab depth: the box may widen by this fra
                                     // (0.35, 0.5, 0.9, 1.2: the iteration 1.5-6% slower)
 constexpr int kStagePasses = 4; // staging covers x-y windows up to 4*256 elements
 constexpr int kStageBatch = 4;  // planes loaded per staging round
-constexpr bool kSlabUncheckedPath = false;  // sampling without the per-sample box test (see slab_ray)
+constexpr bool kSlabUncheckedPath = true;  // sampling without the per-sample box test (see slab_ray)
 
 struct SlabBox {
     int lo[3], n[3];   // box origin and extent per volume axis (x, y, z)
@@ -759,12 +759,20 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     const float a1 = fmaxf(fmaxf(e0 * sAmin, e0 * sAmax), fmaxf(e1 * sAmin, e1 * sAmax)) + o[A];
     const float b0 = fminf(fminf(e0 * sBmin, e0 * sBmax), fminf(e1 * sBmin, e1 * sBmax)) + o[B];
     const float b1 = fmaxf(fmaxf(e0 * sBmin, e0 * sBmax), fmaxf(e1 * sBmin, e1 * sBmax)) + o[B];
-    // cells of the samples' exact positions widened by box_eps, plus the +1 taps
-    const float lim = 1.0e6f, eps = g.box_eps;
-    const int loA = max((int)floorf(fminf(fmaxf(a0 - eps, -lim), lim)), 0);
-    const int loB = max((int)floorf(fminf(fmaxf(b0 - eps, -lim), lim)), 0);
-    const int hiA = min((int)floorf(fminf(fmaxf(a1 + eps, -lim), lim)) + 1, dims[A]);
-    const int hiB = min((int)floorf(fminf(fmaxf(b1 + eps, -lim), lim)) + 1, dims[B]);
+    // Cells of the samples' clamped positions, plus the +1 taps.  A sample
+    // lies within box_eps of its exact line, and the sampler clamps it to
+    // [0, dims-1] before taking its cell; clamping is monotone, so the cells
+    // of the slab's samples lie between the cells of the clamped, widened
+    // extremes.  (Clipping only the integer bounds instead let a block whose
+    // samples all lie beyond a face -- pre-entry samples of grazing rays,
+    // clamped onto it -- get lo = dims above their cell dims-1, or a box one
+    // cell wide at the low face missing the +1 tap.)
+    const float eps = g.box_eps;
+    const float bA = D == 0 ? g.by : g.bx, bB = D == 2 ? g.by : g.bz;
+    const int loA = (int)floorf(fminf(fmaxf(a0 - eps, 0.0f), bA));
+    const int loB = (int)floorf(fminf(fmaxf(b0 - eps, 0.0f), bB));
+    const int hiA = (int)floorf(fminf(fmaxf(a1 + eps, 0.0f), bA)) + 1;
+    const int hiB = (int)floorf(fminf(fmaxf(b1 + eps, 0.0f), bB)) + 1;
     const int nA = hiA - loA + 1;
     const int nB = hiB - loB + 1;
     if (nA > 0 && nB > 0) {
@@ -917,12 +925,6 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     // without a per-sample box test.
     const float cmax = 2.0f * (fmaxf(g.bx, fmaxf(g.by, g.bz)) + 1.0f);
     const bool near = fmaxf(fabsf(G.pend[0]), fmaxf(fabsf(G.pend[1]), fabsf(G.pend[2]))) <= cmax;
-    // The unchecked path is off: at 40^3 on a 384^2 detector of 0.25 mm
-    // pixels, grazing rays near the volume's edge took samples outside the
-    // staged box in it (a few rays per view read other voxels; the checked
-    // path below agrees with the level projector there).  Its containment
-    // bound needs repair before it is used again; it was worth 8% of the
-    // 256^3 x 8 iteration.
     const bool wave_in_box = kSlabUncheckedPath && __builtin_amdgcn_ballot_w64(have && !(fwd && near)) == 0;
     const float maxn = (float)g.max_samples;
     float nf = (float)n;

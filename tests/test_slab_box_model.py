@@ -1,0 +1,67 @@
+"""Host model of the slab projector's staged-box containment
+(tests/slab_box_model.cpp): the sample walk and slab boxes of k_fp_slab with
+the same fp32 operations, geometry from the library's build_view.  Every
+sample that the unchecked sampling path takes must have its clamped cell
+and +1 taps inside the staged box.
+
+The bound the kernel uses (1: line extremes clamped to [0, dims-1] before
+the floor) leaves no sample outside at the geometry where the first bound
+(0) failed on the device -- 40^3 on a 384^2 detector of 0.25 mm pixels,
+where blocks of grazing rays had all their samples in a slab beyond a face --
+nor at the other geometries below; the first bound's misses there are
+reproduced, so the model sees the failure it guards against."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import spacetimetomography_amd as stt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def model(tmp_path_factory):
+    so = str(tmp_path_factory.mktemp("slabmodel") / "slab_model.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-o", so,
+                    os.path.join(HERE, "slab_box_model.cpp"),
+                    os.path.join(ROOT, "spacetimetomography_amd", "csrc", "stt_geometry.cpp")], check=True)
+    lib = C.CDLL(so)
+    lib.slab_model.restype = C.c_int
+    lib.slab_model.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_longlong)]
+
+    def run(vol, det, spacing, ds, sid, sdd, angles, bound, oxyz=(0, 0, 0)):
+        g = stt.Geometry.make(vol, det, spacing, ds, sid, sdd, oxyz)
+        a = np.ascontiguousarray(angles, np.float32)
+        out = (C.c_longlong * 4)()
+        assert lib.slab_model(C.byref(g), a.ctypes.data_as(C.POINTER(C.c_float)), len(a), bound, out) == 0
+        return dict(checked=out[0], outside=out[1], waves=out[2], samples=out[3])
+
+    return run
+
+
+def test_fine_detector_grazing_rays(model):
+    ang = list(stt.synth_angles(2, 12, offset=0))
+    old = model((40, 40, 40), (384, 384), (1, 1, 1), 0.25, 600.0, 1000.0, ang, 0)
+    new = model((40, 40, 40), (384, 384), (1, 1, 1), 0.25, 600.0, 1000.0, ang, 1)
+    print(old, new)
+    assert old["outside"] > 0           # the failure seen on the device
+    assert new["outside"] == 0
+    assert new["checked"] > 0.99 * new["samples"]  # nearly every sample takes the unchecked path
+    assert new["samples"] == old["samples"]
+
+
+@pytest.mark.parametrize("case", [
+    ((64, 48, 56), (200, 160), (1.0, 1.0, 1.0), 0.5, 120.0, 200.0, (0.0, 0.0, 0.0)),
+    ((40, 36, 44), (128, 112), (0.9, 1.2, 1.0), 0.6, 80.0, 140.0, (2.0, -1.5, 1.0)),
+    ((32, 32, 32), (96, 96), (1.0, 1.0, 1.0), 0.2, 60.0, 100.0, (0.0, 3.0, 0.0)),
+])
+def test_other_geometries(model, case):
+    vol, det, sp, ds, sid, sdd, oxyz = case
+    ang = list(stt.synth_angles(1, 10, offset=3)) + [0.0, 45.0, 90.0, 135.0, 180.0, 270.0]
+    new = model(vol, det, sp, ds, sid, sdd, ang, 1, oxyz)
+    print(new)
+    assert new["outside"] == 0 and new["checked"] > 0
