@@ -700,7 +700,6 @@ constexpr int kSlabKMin = 8;    // fewest sample planes per slab (before the LDS
 constexpr float kSlabShear = 0.7f;  // slab depth: the box may widen by this fraction of its near-plane width
                                     // (0.35, 0.5, 0.9, 1.2: the iteration 1.5-6% slower)
 constexpr int kStagePasses = 4; // staging covers x-y windows up to 4*256 elements
-constexpr int kStageBatch = 4;  // planes loaded per staging round
 constexpr bool kSlabUncheckedPath = true;  // sampling without the per-sample box test (see slab_ray)
 
 struct SlabBox {
@@ -807,53 +806,55 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     bx.px = bx.n[0] | 1;
     bx.pxy = (bx.px * bx.n[1]) | 1;
     bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap &&
-                bx.n[0] * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
+                bx.px * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
     bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
     return bx;
 }
 
-// Cooperative load of the box into LDS.  Each thread owns fixed positions
-// (column, row) of the box's x-y window -- element tid + NT*m of it -- and
-// walks the z planes, so a plane step moves only wave-uniform offsets.
-// kStageBatch loads are in flight before their LDS writes.  Indices up to dims (the
-// +1 taps of the last voxel, weight 0) read the volume's zero tail or the
-// next row, never past the allocation.
+// Cooperative load of the box into LDS with gfx950's direct-to-LDS buffer
+// loads (buffer_load_dword ... lds): each lane's dword lands at the wave's
+// LDS base (M0) + 4 * lane, with no register round trip and no LDS write
+// instruction.  The box's x-y window is linearised with the LDS row pitch px
+// (element e = col + px * row, col < px), so the 64 lanes of a wave holding
+// window elements e0 .. e0+63 fill 64 consecutive LDS dwords of a plane.
+// Each thread owns fixed window elements (tid + NT*m) and issues the loads
+// of all the box's planes back to back (plane steps move only the
+// wave-uniform offsets); the caller waits for them and synchronises before
+// sampling.  The padding column (col = n[0] when n[0] is even) loads the next
+// voxel along x, and indices up to dims (the +1 taps of the last voxel,
+// weight 0) read the volume's zero tail or the next row: every address stays
+// inside the allocation, which the buffer range check also enforces.
 template <int NT>
 __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, const SlabBox& bx, float* box) {
     const int tid = threadIdx.x;
-    const int nbx = bx.n[0], nby = bx.n[1], nbz = bx.n[2];
-    const int npl = nbx * nby;
-    // e / nbx for e < 2^16; a one-column box (no sample of the block in the
-    // slab) would overflow the multiplier, so its rows are taken directly
-    const unsigned magic = nbx > 1 ? (unsigned)(0xFFFFFFFFull / (unsigned)nbx) + 1u : 0u;
+    const int wave0 = __builtin_amdgcn_readfirstlane(tid & ~63);
+    const int px = bx.px, nbz = bx.n[2];
+    const int npl = px * bx.n[1];
+    // e / px for e < 2^16 (px == 1: the rows are taken directly)
+    const unsigned magic = px > 1 ? (unsigned)(0xFFFFFFFFull / (unsigned)px) + 1u : 0u;
     const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
     const unsigned zbase = 4u * nxy * (unsigned)bx.lo[2];
 #pragma unroll
     for (int m = 0; m < kStagePasses * 256 / NT; ++m) {
         if (NT * m >= npl) break;  // uniform
         const int e = tid + NT * m;
-        const bool valid = e < npl;
-        const unsigned row = nbx > 1 ? __umulhi((unsigned)e, magic) : (unsigned)e;
-        const unsigned col = (unsigned)e - row * (unsigned)nbx;
+        const unsigned row = px > 1 ? __umulhi((unsigned)e, magic) : (unsigned)e;
+        const unsigned col = (unsigned)e - row * (unsigned)px;
         const unsigned voff = 4u * ((unsigned)bx.lo[0] + col + nx * ((unsigned)bx.lo[1] + row));
-        const int loff = (int)col + bx.px * (int)row;
-        if (valid) {
-            // kStageBatch planes per round, branch-free: the last round
-            // repeats plane nbz-1 instead of testing each plane (uniform
-            // branches around every load and write cost 1.2 ms per
-            // iteration at 256^3 x 8; batches of 3, 5-8 and 12 measured slower)
-            for (int z0 = 0; z0 < nbz; z0 += kStageBatch) {
-                float vals[kStageBatch];
-#pragma unroll
-                for (int k = 0; k < kStageBatch; ++k)
-                    vals[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                            vv.rsrc, voff, zbase + 4u * nxy * (unsigned)min(z0 + k, nbz - 1), 0));
-#pragma unroll
-                for (int k = 0; k < kStageBatch; ++k) box[loff + bx.pxy * min(z0 + k, nbz - 1)] = vals[k];
-            }
+        if (e < npl) {
+            typedef __attribute__((address_space(3))) void lds_void;
+            lds_void* dst = (lds_void*)(box + wave0 + NT * m);
+#pragma unroll 4
+            for (int z = 0; z < nbz; ++z)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(vv.rsrc, (lds_void*)((float __attribute__((address_space(3)))*)dst + bx.pxy * z),
+                                                         4, voff, zbase + 4u * nxy * (unsigned)z, 0, 0);
         }
     }
 }
+
+// Waits for the direct-to-LDS loads of slab_stage (vmcnt(0); expcnt and
+// lgkmcnt left at their maxima).
+__device__ __forceinline__ void slab_stage_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 template <int D, bool DOWN, int NT>
 __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
@@ -935,6 +936,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         SlabBox bx = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3], dfirst);
         if (!slopes_ok) bx.staged = false;
         if (bx.staged) slab_stage<NT>(vv, g, bx, box);
+        slab_stage_wait();
         __syncthreads();
         if (bx.staged && wave_in_box) {
             // Samples of the slab are those whose clamped base plane p = (int)cD
