@@ -710,6 +710,35 @@ struct SlabBox {
     bool staged;
 };
 
+// A slab box passes from the wave that computes it to the whole block
+// through an LDS slot of kBoxInts ints, read back as wave-uniform values.
+constexpr int kBoxInts = 14;
+__device__ __forceinline__ void box_put(int* slot, const SlabBox& b) {
+    if ((threadIdx.x & 63) == 0) {
+        const int v[kBoxInts] = {b.lo[0], b.lo[1], b.lo[2], b.n[0], b.n[1], b.n[2], b.cbeg,
+                                 b.cend,  b.px,    b.pxy,   b.base, (int)b.staged, 0, 0};
+#pragma unroll
+        for (int k = 0; k < kBoxInts; ++k) slot[k] = v[k];
+    }
+}
+__device__ __forceinline__ SlabBox box_get(const int* slot) {
+    int v[kBoxInts];
+#pragma unroll
+    for (int k = 0; k < kBoxInts; ++k) v[k] = __builtin_amdgcn_readfirstlane(slot[k]);
+    SlabBox b;
+    for (int c = 0; c < 3; ++c) {
+        b.lo[c] = v[c];
+        b.n[c] = v[3 + c];
+    }
+    b.cbeg = v[6];
+    b.cend = v[7];
+    b.px = v[8];
+    b.pxy = v[9];
+    b.base = v[10];
+    b.staged = v[11] != 0;
+    return b;
+}
+
 // Block-wide min/max of NV values (wave shuffles, then one LDS round).
 template <int NW, int NV>
 __device__ __forceinline__ void block_minmax(float v[NV], const bool is_max[NV], float* red) {
@@ -858,7 +887,7 @@ __device__ __forceinline__ void slab_stage_wait() { __builtin_amdgcn_s_waitcnt(0
 
 template <int D, bool DOWN, int NT>
 __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
-                                          float* box, float* red) {
+                                          float* box, float* red, int* slots) {
     constexpr int A = D == 0 ? 1 : 0;
     constexpr int B = D == 2 ? 1 : 2;
     const VolView vv = vol_view(it.vol, g);
@@ -929,13 +958,31 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     const bool wave_in_box = kSlabUncheckedPath && __builtin_amdgcn_ballot_w64(have && !(fwd && near)) == 0;
     const float maxn = (float)g.max_samples;
     float nf = (float)n;
+    // The boxes are uniform arithmetic on the block's slopes, which gfx950
+    // can only do on the vector ALUs: one wave computes each box and hands it
+    // to the others through an LDS slot.  The first slab's box (which alone
+    // reaches back to samples before an entry point, dfirst) is computed
+    // here; the box of slab s+1 depends only on where slab s ends, so wave
+    // (s mod NT/64) computes it while slab s is sampled, and the barrier that
+    // ends slab s publishes it.
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (any_ray) {
+        if (wv == 0) {
+            SlabBox b0 = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3], q[5]);
+            if (!slopes_ok) b0.staged = false;
+            box_put(slots, b0);
+        }
+        __syncthreads();
+    }
 #pragma unroll 1
     for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
-        // only the first slab holds samples before an entry point
-        const float dfirst = guard == 0 ? q[5] : (DOWN ? -INFINITY : INFINITY);
-        SlabBox bx = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3], dfirst);
-        if (!slopes_ok) bx.staged = false;
+        const SlabBox bx = box_get(slots + kBoxInts * (guard & 1));
         if (bx.staged) slab_stage<NT>(vv, g, bx, box);
+        if (wv == (guard & (NT / 64 - 1))) {
+            SlabBox nb = slab_box<D, DOWN>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);
+            if (!slopes_ok) nb.staged = false;
+            box_put(slots + kBoxInts * ((guard + 1) & 1), nb);
+        }
         slab_stage_wait();
         __syncthreads();
         if (bx.staged && wave_in_box) {
@@ -1041,6 +1088,7 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     constexpr int NT = TW * TH;
     __shared__ float box[kSlabCap];
     __shared__ float red[6 * NT / 64];
+    __shared__ int slots[2 * kBoxInts];
     int i, j;
     const bool valid = tile_pixel<TW, TH>(g, i, j);
     const WorkItem& it = items[blockIdx.y];
@@ -1052,12 +1100,12 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     int ns = 0;
     float value;
     switch (march) {
-        case 0: value = slab_ray<0, false, NT>(it, g, r, hit, ns, box, red); break;
-        case 4: value = slab_ray<0, true, NT>(it, g, r, hit, ns, box, red); break;
-        case 1: value = slab_ray<1, false, NT>(it, g, r, hit, ns, box, red); break;
-        case 5: value = slab_ray<1, true, NT>(it, g, r, hit, ns, box, red); break;
-        case 6: value = slab_ray<2, true, NT>(it, g, r, hit, ns, box, red); break;
-        default: value = slab_ray<2, false, NT>(it, g, r, hit, ns, box, red); break;
+        case 0: value = slab_ray<0, false, NT>(it, g, r, hit, ns, box, red, slots); break;
+        case 4: value = slab_ray<0, true, NT>(it, g, r, hit, ns, box, red, slots); break;
+        case 1: value = slab_ray<1, false, NT>(it, g, r, hit, ns, box, red, slots); break;
+        case 5: value = slab_ray<1, true, NT>(it, g, r, hit, ns, box, red, slots); break;
+        case 6: value = slab_ray<2, true, NT>(it, g, r, hit, ns, box, red, slots); break;
+        default: value = slab_ray<2, false, NT>(it, g, r, hit, ns, box, red, slots); break;
     }
     if (!valid) return;
     const int k = i + j * g.W;
