@@ -885,7 +885,9 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
 // lgkmcnt left at their maxima).
 __device__ __forceinline__ void slab_stage_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-template <int D, bool DOWN, int NT>
+// COUNT: the sample count is an output (FP mode); the SART pass needs only
+// the line integral, so its unchecked loop keeps no count.
+template <int D, bool DOWN, int NT, bool COUNT>
 __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
                                           float* box, float* red, int* slots) {
     constexpr int A = D == 0 ? 1 : 0;
@@ -990,17 +992,26 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
             // satisfies p < cend (up) or p > cend (down); cD >= 0, so the test
             // runs on the float.  A lane leaves the loop at its first sample
             // past the slab, or at max_samples, or when its ray ends (nlim = -1).
+            // Without COUNT, an ended ray's lane gets an unreachable plane
+            // limit instead (the loop still ends: every sample advances t
+            // towards a finite segment end).
             const float lim = DOWN ? (float)(bx.cend + 1) : (float)bx.cend;
+            constexpr float kEnded = DOWN ? INFINITY : -INFINITY;
             const char* bb = (const char*)(box - bx.base);
             const unsigned px4 = 4u * (unsigned)bx.px, pxy4 = 4u * (unsigned)bx.pxy;
             float nlim = have ? maxn : -1.0f;
+            float llim = have ? lim : kEnded;
 #pragma unroll 1
             for (;;) {
                 const float cx = __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx);
                 const float cy = __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by);
                 const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
                 const float cc[3] = {cx, cy, cz};
-                if (!((DOWN ? cc[D] >= lim : cc[D] < lim) && nf < nlim)) break;
+                if (COUNT) {
+                    if (!((DOWN ? cc[D] >= lim : cc[D] < lim) && nf < nlim)) break;
+                } else if (!(DOWN ? cc[D] >= llim : cc[D] < llim)) {
+                    break;
+                }
                 const unsigned off = ((unsigned)(int)cx << 2) + __umul24((unsigned)(int)cy, px4) +
                                      __umul24((unsigned)(int)cz, pxy4);
                 const float* b0 = (const float*)(bb + off);
@@ -1012,17 +1023,20 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
                 T.r10 = make_float2(b0[bx.px], b0[bx.px + 1]);
                 T.r01 = make_float2(b0[bx.pxy], b0[bx.pxy + 1]);
                 T.r11 = make_float2(b0[bx.pxy + bx.px], b0[bx.pxy + bx.px + 1]);
-                nf += 1.0f;
+                if (COUNT) nf += 1.0f;
                 G.t += sdw;
                 G.pend[0] += G.step[0];
                 G.pend[1] += G.step[1];
                 G.pend[2] += G.step[2];
                 if (__builtin_expect(!(G.t < G.tb), 0)) {
-                    if (!raygen_next_segment(G, sdw)) nlim = -1.0f;
+                    if (!raygen_next_segment(G, sdw)) {
+                        nlim = -1.0f;
+                        llim = kEnded;
+                    }
                 }
                 density = fmaf(lerp_taps(T), sdw, density);
             }
-            have = nf < nlim;
+            have = COUNT ? nf < nlim : llim != kEnded;
             c0 = bx.cend;
             if (!__syncthreads_or(have)) break;
             continue;
@@ -1100,12 +1114,12 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     int ns = 0;
     float value;
     switch (march) {
-        case 0: value = slab_ray<0, false, NT>(it, g, r, hit, ns, box, red, slots); break;
-        case 4: value = slab_ray<0, true, NT>(it, g, r, hit, ns, box, red, slots); break;
-        case 1: value = slab_ray<1, false, NT>(it, g, r, hit, ns, box, red, slots); break;
-        case 5: value = slab_ray<1, true, NT>(it, g, r, hit, ns, box, red, slots); break;
-        case 6: value = slab_ray<2, true, NT>(it, g, r, hit, ns, box, red, slots); break;
-        default: value = slab_ray<2, false, NT>(it, g, r, hit, ns, box, red, slots); break;
+        case 0: value = slab_ray<0, false, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
+        case 4: value = slab_ray<0, true, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
+        case 1: value = slab_ray<1, false, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
+        case 5: value = slab_ray<1, true, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
+        case 6: value = slab_ray<2, true, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
+        default: value = slab_ray<2, false, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
     }
     if (!valid) return;
     const int k = i + j * g.W;
