@@ -168,7 +168,11 @@ int  stt_backproject_voxel(stt_ctx* ctx, int frame, int view, const float* corr,
                            double alpha);
 /* Ray-driven backprojection (ST_Tomography.cpp:733-765 with the scatter of
  * SingleScalarVolume.core.h:128-143): x += contrib/wsum, clamp (0/0 -> 0).
- * contrib/wsum may be NULL; when given they receive the accumulators. */
+ * The sums are exact 64-bit fixed-point accumulations of the reference's
+ * fp32 addends (identical on every run), divided in FP64.  contrib/wsum may
+ * be NULL; when given they receive the sums rounded to fp32.  In a sweep or
+ * iteration the ray lengths are reset after every ray-BP view, as the
+ * reference's TypicalRayProducer::Initial() does. */
 int  stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr,
                          float* contrib, float* wsum);
 /* Matched transpose of stt_forward_project: out += A^T y (the exact adjoint of
@@ -234,7 +238,8 @@ typedef struct stt_group stt_group;
 int  stt_group_create(stt_ctx* const* ctxs, int n, stt_group** out);
 int  stt_group_destroy(stt_group* grp);
 int  stt_group_iterate(stt_group* grp, int n_iters, int use_graph, double* total_ms);
-/* Projector shape (process-wide): 0 = slab projector (16x16 rays per block,
+/* Projector shape of this context (other contexts keep theirs; the context's
+ * captured graph is rebuilt): 0 = slab projector (16x16 rays per block,
  * volume boxes staged in LDS; default); 64, 32, 16 or 8 = level-synchronous
  * projector with that many rays per detector row of a 64-lane wave. */
 int  stt_set_projector_tile(stt_ctx* ctx, int rays_per_row);

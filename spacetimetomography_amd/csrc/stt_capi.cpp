@@ -73,6 +73,19 @@ struct DevMem {
         if (p) (void)hipFree(p);
     }
 };
+// Stream-ordered scratch: freed with hipFreeAsync on the same stream on
+// every return path, after the work queued before the return.
+struct StreamMem {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    hipError_t alloc(size_t bytes, hipStream_t stream) {
+        s = stream;
+        return hipMallocAsync(&p, bytes, stream);
+    }
+    ~StreamMem() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
 struct OwnedStream {
     hipStream_t s = nullptr;
     ~OwnedStream() {
@@ -1147,7 +1160,9 @@ int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* ray
     if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
     const size_t D = ctx->D;
     float *dax = nullptr, *drl = nullptr, *dw = nullptr;
-    HIPCHK(hipMallocAsync((void**)&dax, 3 * D * sizeof(float), ctx->stream));
+    StreamMem dax_mem;
+    HIPCHK(dax_mem.alloc(3 * D * sizeof(float), ctx->stream));
+    dax = (float*)dax_mem.p;
     drl = dax + D;
     dw = drl + D;
     if (raylen) HIPCHK(hipMemcpyAsync(drl, raylen, D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
@@ -1162,7 +1177,6 @@ int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* ray
     if (Ax) HIPCHK(hipMemcpyAsync(Ax, dax, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     if (raylen) HIPCHK(hipMemcpyAsync(raylen, drl, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     if (weight) HIPCHK(hipMemcpyAsync(weight, dw, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipFreeAsync(dax, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
@@ -1232,7 +1246,9 @@ int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
     if (!y || !out) return fail(STT_ERR_INVALID, "null argument");
     if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
     float* acc = nullptr;
-    HIPCHK(hipMallocAsync((void**)&acc, (ctx->Vpad + ctx->D) * sizeof(float), ctx->stream));
+    StreamMem acc_mem;
+    HIPCHK(acc_mem.alloc((ctx->Vpad + ctx->D) * sizeof(float), ctx->stream));
+    acc = (float*)acc_mem.p;
     HIPCHK(hipMemsetAsync(acc, 0, ctx->Vpad * sizeof(float), ctx->stream));
     HIPCHK(hipMemcpyAsync(acc, out, ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     float* dy = acc + ctx->Vpad;
@@ -1246,7 +1262,6 @@ int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
     // A x = sd * sum_s trilinear(x, p_s), so A^T y scatters y*sd with the trilinear weights.
     HIPCHK(launch_fp(FP_MODE_SCATTER, ctx->items_scratch, 1, ctx->vg, ctx->vg.sd, ctx->stream));
     HIPCHK(hipMemcpyAsync(out, acc, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipFreeAsync(acc, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
@@ -1286,13 +1301,14 @@ int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* 
         return fail(STT_ERR_INVALID, "unknown interpolation");
     const size_t V = ctx->V;
     float* buf = nullptr;
-    HIPCHK(hipMallocAsync((void**)&buf, 5 * V * sizeof(float), ctx->stream));
+    StreamMem buf_mem;
+    HIPCHK(buf_mem.alloc(5 * V * sizeof(float), ctx->stream));
+    buf = (float*)buf_mem.p;
     HIPCHK(hipMemcpyAsync(buf, uvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(buf + 3 * V, src, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(launch_warp(buf, buf + 3 * V, buf + 4 * V, ctx->vg, dt, interp == STT_INTERP_CUBIC_CLAMPED,
                        ctx->stream));
     HIPCHK(hipMemcpyAsync(out, buf + 4 * V, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipFreeAsync(buf, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
@@ -1305,11 +1321,12 @@ int stt_warp_frame(stt_ctx* ctx, int pair, float dt, int interp, float* out) {
         return fail(STT_ERR_INVALID, "unknown interpolation");
     const size_t V = ctx->V;
     float* buf = nullptr;
-    HIPCHK(hipMallocAsync((void**)&buf, V * sizeof(float), ctx->stream));
+    StreamMem buf_mem;
+    HIPCHK(buf_mem.alloc(V * sizeof(float), ctx->stream));
+    buf = (float*)buf_mem.p;
     HIPCHK(launch_warp(ctx->flow[pair], ctx->x[pair + 1], buf, ctx->vg, dt, interp == STT_INTERP_CUBIC_CLAMPED,
                        ctx->stream));
     HIPCHK(hipMemcpyAsync(out, buf, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipFreeAsync(buf, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
@@ -1321,7 +1338,9 @@ int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, con
     if (!gsrc && !guvw) return STT_OK;
     const size_t V = ctx->V;
     float* buf = nullptr;
-    HIPCHK(hipMallocAsync((void**)&buf, 9 * V * sizeof(float), ctx->stream));
+    StreamMem buf_mem;
+    HIPCHK(buf_mem.alloc(9 * V * sizeof(float), ctx->stream));
+    buf = (float*)buf_mem.p;
     float *d_uvw = buf, *d_src = buf + 3 * V, *d_g = buf + 4 * V, *d_gsrc = buf + 5 * V, *d_guvw = buf + 6 * V;
     HIPCHK(hipMemcpyAsync(d_uvw, uvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(d_src, src, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
@@ -1332,7 +1351,6 @@ int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, con
                            ctx->stream));
     if (gsrc) HIPCHK(hipMemcpyAsync(gsrc, d_gsrc, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     if (guvw) HIPCHK(hipMemcpyAsync(guvw, d_guvw, 3 * V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipFreeAsync(buf, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
@@ -1547,7 +1565,7 @@ int stt_set_projector_tile(stt_ctx* ctx, int rays_per_row) {
     CTX_CHECK(ctx);
     if (rays_per_row != 0 && rays_per_row != 64 && rays_per_row != 32 && rays_per_row != 16 && rays_per_row != 8)
         return fail(STT_ERR_INVALID, "projector shape must be 0 (slab), 64, 32, 16 or 8");
-    set_fp_tile(rays_per_row);
+    ctx->vg.fp_tile = rays_per_row;
     ctx->invalidate_graph();
     return STT_OK;
 }
@@ -1572,7 +1590,9 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
         cnt.push_back((int)items.size() - off.back());
     }
     float* buf = nullptr;
-    HIPCHK(hipMallocAsync((void**)&buf, 2 * items.size() * D * sizeof(float), ctx->stream));
+    StreamMem buf_mem;
+    HIPCHK(buf_mem.alloc(2 * items.size() * D * sizeof(float), ctx->stream));
+    buf = (float*)buf_mem.p;
     for (size_t n = 0; n < items.size(); ++n) {
         items[n].corr = buf + 2 * n * D;
         items[n].raylen = buf + (2 * n + 1) * D;
@@ -1580,13 +1600,12 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     HIPCHK(hipMemsetAsync(buf, 0, 2 * items.size() * D * sizeof(float), ctx->stream));
     rc = ctx->upload_scratch(items);
     if (rc) {
-        (void)hipFreeAsync(buf, ctx->stream);
         return rc;
     }
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const int old = fp_tile();
+    const int old = ctx->vg.fp_tile;
     int best = old;
     double best_ms = 1e30;
     auto sweep = [&]() -> hipError_t {
@@ -1599,7 +1618,7 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
         return hipSuccess;
     };
     for (int si = 0; si < 5; ++si) {
-        set_fp_tile(shapes[si]);
+        ctx->vg.fp_tile = shapes[si];
         HIPCHK(sweep());  // warm-up
         HIPCHK(hipEventRecord(e0, ctx->stream));
         for (int rep = 0; rep < 2; ++rep) HIPCHK(sweep());
@@ -1616,9 +1635,8 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    set_fp_tile(best);
+    ctx->vg.fp_tile = best;
     if (best != old) ctx->invalidate_graph();
-    HIPCHK(hipFreeAsync(buf, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (rays_per_row) *rays_per_row = best;
     return STT_OK;
@@ -1689,9 +1707,15 @@ int stt_residual_norm(stt_ctx* ctx, double* r2) {
     float* ax = nullptr;
     double* partial = nullptr;
     WorkItem* dev_items = nullptr;
-    HIPCHK(hipMallocAsync((void**)&ax, 2 * (size_t)F * D * sizeof(float), ctx->stream));
-    HIPCHK(hipMallocAsync((void**)&partial, ((size_t)nitems * kReductionBlocks + 1) * sizeof(double), ctx->stream));
-    HIPCHK(hipMallocAsync((void**)&dev_items, (size_t)nitems * sizeof(WorkItem), ctx->stream));
+    StreamMem ax_mem;
+    HIPCHK(ax_mem.alloc(2 * (size_t)F * D * sizeof(float), ctx->stream));
+    ax = (float*)ax_mem.p;
+    StreamMem partial_mem;
+    HIPCHK(partial_mem.alloc(((size_t)nitems * kReductionBlocks + 1) * sizeof(double), ctx->stream));
+    partial = (double*)partial_mem.p;
+    StreamMem dev_items_mem;
+    HIPCHK(dev_items_mem.alloc((size_t)nitems * sizeof(WorkItem), ctx->stream));
+    dev_items = (WorkItem*)dev_items_mem.p;
     for (int k = 0; k < ctx->maxP; ++k)
         for (int n = 0; n < cnt[k]; ++n) {
             WorkItem& it = items[off[k] + n];
@@ -1711,9 +1735,6 @@ int stt_residual_norm(stt_ctx* ctx, double* r2) {
     double* out = partial + (size_t)nitems * kReductionBlocks;
     HIPCHK(launch_sum_partials(partial, nitems * kReductionBlocks, out, ctx->stream));
     HIPCHK(hipMemcpyAsync(r2, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipFreeAsync(ax, ctx->stream));
-    HIPCHK(hipFreeAsync(partial, ctx->stream));
-    HIPCHK(hipFreeAsync(dev_items, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
@@ -1768,9 +1789,15 @@ int stt_power_iteration(stt_ctx* ctx, int iters, double* L) {
     const size_t V = ctx->V, n = (size_t)F * V;
     float *v = nullptr, *y = nullptr;
     double* red = nullptr;
-    HIPCHK(hipMallocAsync((void**)&v, 2 * n * sizeof(float), ctx->stream));  // v, then w = K^T K v
-    HIPCHK(hipMallocAsync((void**)&y, 4 * n * sizeof(float), ctx->stream));
-    HIPCHK(hipMallocAsync((void**)&red, (kReductionBlocks + 1) * sizeof(double), ctx->stream));
+    StreamMem v_mem;
+    HIPCHK(v_mem.alloc(2 * n * sizeof(float), ctx->stream));
+    v = (float*)v_mem.p;
+    StreamMem y_mem;
+    HIPCHK(y_mem.alloc(4 * n * sizeof(float), ctx->stream));
+    y = (float*)y_mem.p;
+    StreamMem red_mem;
+    HIPCHK(red_mem.alloc((kReductionBlocks + 1) * sizeof(double), ctx->stream));
+    red = (double*)red_mem.p;
     float* w = v + n;
     double* nrm2 = red + kReductionBlocks;
     auto norm2 = [&](const float* a) -> hipError_t {
@@ -1790,10 +1817,6 @@ int stt_power_iteration(stt_ctx* ctx, int iters, double* L) {
     }
     double lam2 = 0.0;
     HIPCHK(hipMemcpyAsync(&lam2, nrm2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipFreeAsync(std::min(v, w), ctx->stream));
-    HIPCHK(hipFreeAsync(y, ctx->stream));
-    HIPCHK(hipFreeAsync(red, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *L = std::sqrt(std::sqrt(lam2));  // lambda_max(K^T K) = ||K^T K v||; ||K|| = sqrt of it
     return STT_OK;

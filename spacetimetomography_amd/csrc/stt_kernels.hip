@@ -2395,16 +2395,12 @@ static inline dim3 vol_grid(const VolGeom& g, int nitems) {
     return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nitems);
 }
 
-// Projector shape, set by stt_set_projector_tile() / the tuner: 0 = slab
-// projector (16x16 rays per block, LDS-staged volume boxes); 64, 32, 16, 8 =
-// level-synchronous projector with that many rays per wave row (64x4, 32x8,
-// 16x16 or 8x32 rays per 256-thread block, transposed copy for x-dominant
-// views in the SART sweep).
-static int g_fp_tile = 0;
-
-void set_fp_tile(int tx) { g_fp_tile = (tx == 64 || tx == 32 || tx == 16 || tx == 8) ? tx : 0; }
-int fp_tile() { return g_fp_tile; }
-bool fp_needs_transpose() { return g_fp_tile != 0; }
+// Projector shape, VolGeom::fp_tile of the launching context (set by
+// stt_set_projector_tile() / the tuner): 0 = slab projector (16x16 rays per
+// block, LDS-staged volume boxes); 64, 32, 16, 8 = level-synchronous
+// projector with that many rays per wave row (64x4, 32x8, 16x16 or 8x32 rays
+// per 256-thread block, transposed copy for x-dominant views in the SART
+// sweep).
 
 template <int MODE>
 static hipError_t launch_fp_slab(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
@@ -2415,7 +2411,7 @@ static hipError_t launch_fp_slab(const WorkItem* items, int nitems, const VolGeo
 
 template <int MODE>
 static hipError_t launch_fp_level(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
-    const int tx = g_fp_tile, ty = 256 / tx;
+    const int tx = g.fp_tile, ty = 256 / tx;
     const dim3 grid(((g.W + tx - 1) / tx) * ((g.H + ty - 1) / ty), nitems);
     switch (tx) {
         case 64: hipLaunchKernelGGL((k_fp_level<MODE, 64>), grid, dim3(256), 0, s, items, g, factor); break;
@@ -2432,10 +2428,10 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
     dim3 grid(nb, nitems);
     switch (mode) {
         case FP_MODE_FP:
-            return g_fp_tile == 0 ? launch_fp_slab<FP_MODE_FP>(items, nitems, g, factor, s)
+            return g.fp_tile == 0 ? launch_fp_slab<FP_MODE_FP>(items, nitems, g, factor, s)
                                   : launch_fp_level<FP_MODE_FP>(items, nitems, g, factor, s);
         case FP_MODE_SART:
-            return g_fp_tile == 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
+            return g.fp_tile == 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
                                   : launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
         case FP_MODE_RAYBP: hipLaunchKernelGGL((k_fp<FP_MODE_RAYBP, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
@@ -2445,7 +2441,7 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s) {
-    if (!fp_needs_transpose()) return hipSuccess;
+    if (g.fp_tile == 0) return hipSuccess;
     const dim3 grid((g.nx + 31) / 32, (g.nz + 31) / 32, g.ny * nitems);
     hipLaunchKernelGGL(k_transpose_xz, grid, dim3(256), 0, s, items, g);
     return hipGetLastError();

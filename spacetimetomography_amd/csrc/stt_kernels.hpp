@@ -14,11 +14,8 @@ enum { FP_MODE_FP = 0, FP_MODE_SART = 1, FP_MODE_SCATTER = 2, FP_MODE_RAYBP = 3 
 //   FP_MODE_SCATTER : contrib += factor*corr*w (fp32 atomics; the projector's transpose)
 //   FP_MODE_RAYBP   : acc_c += corr*w, acc_w += w in exact 64-bit fixed point
 //                     (ray BP; launch_raybp_prep first sets the scale)
-// Projector shape: 0 = slab projector (LDS-staged boxes, default); 64, 32,
-// 16 or 8 = level-synchronous projector with that many rays per wave row.
-void set_fp_tile(int tx);
-int fp_tile();
-bool fp_needs_transpose();
+// Projector shape: g.fp_tile, per context (0 = slab projector, default; 64,
+// 32, 16 or 8 = level-synchronous projector with that many rays per wave row).
 hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s);
 // Refreshes the transposed volume copy of every item whose view samples it
 // (level-synchronous projector only; a no-op for the slab projector).
