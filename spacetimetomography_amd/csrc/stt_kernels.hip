@@ -354,16 +354,23 @@ __device__ __forceinline__ float cast_ray(const VolView& vol, const Sink& sink, 
     return density;
 }
 
-// Detector tile of TX x TY rays per 256-thread block.  Blocks are dealt to the
-// 8 XCDs round-robin; remapping makes each XCD own a contiguous band of
-// detector rows, so its L2 sees one slab of the volume.
+// Detector tile of TX x TY rays per block and the work item (frame) it
+// belongs to; the grid is (tiles, items).  Blocks are dealt to the 8 XCDs
+// round-robin in dispatch order (x fastest); the remapping gives each XCD a
+// contiguous eighth of the (item, tile) range, so an XCD's L2 serves whole
+// frames of a multi-frame launch (and a contiguous band of detector rows,
+// one slab of the volume, of a single-frame launch).  8 frames of 256^3 in
+// one launch.
 template <int TX, int TY>
-__device__ __forceinline__ bool tile_pixel(const VolGeom& g, int& i, int& j) {
+__device__ __forceinline__ bool tile_pixel(const VolGeom& g, int& i, int& j, int& item) {
     const int tiles_x = (g.W + TX - 1) / TX;
     const int tiles_y = (g.H + TY - 1) / TY;
     const int nb = tiles_x * tiles_y;
-    int b = blockIdx.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int n = nb * (int)gridDim.y;
+    int b = (int)blockIdx.x + nb * (int)blockIdx.y;
+    if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
+    item = b / nb;
+    b -= item * nb;
     const int tx = b % tiles_x, ty = b / tiles_x;
     i = tx * TX + (threadIdx.x % TX);
     j = ty * TY + (threadIdx.x / TX);
@@ -373,8 +380,9 @@ __device__ __forceinline__ bool tile_pixel(const VolGeom& g, int& i, int& j) {
 template <int MODE, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_fp(const WorkItem* __restrict__ items, VolGeom g, float factor) {
     int i, j;
-    if (!tile_pixel<TX, TY>(g, i, j)) return;
-    const WorkItem& it = items[blockIdx.y];
+    int item;
+    if (!tile_pixel<TX, TY>(g, i, j, item)) return;
+    const WorkItem& it = items[item];
     const ViewRec& v = *it.view;
     const int k = i + j * g.W;
     Ray r;
@@ -636,8 +644,9 @@ template <int MODE, int TX>
 __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ items, VolGeom g, float factor) {
     constexpr int TY = 256 / TX;
     int i, j;
-    const bool valid = tile_pixel<TX, TY>(g, i, j);
-    const WorkItem& it = items[blockIdx.y];
+    int item;
+    const bool valid = tile_pixel<TX, TY>(g, i, j, item);
+    const WorkItem& it = items[item];
     const ViewRec& v = *it.view;
     const int march = v.march;
     const bool down = (march & 4) != 0;
@@ -1104,8 +1113,9 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     __shared__ float red[6 * NT / 64];
     __shared__ int slots[2 * kBoxInts];
     int i, j;
-    const bool valid = tile_pixel<TW, TH>(g, i, j);
-    const WorkItem& it = items[blockIdx.y];
+    int item;
+    const bool valid = tile_pixel<TW, TH>(g, i, j, item);
+    const WorkItem& it = items[item];
     const ViewRec& v = *it.view;
     const int march = v.march & 7;
     Ray r;
