@@ -576,6 +576,21 @@ def main():
     ctx.synchronize()
     log(f"kernel timing: {ksteps} eager iterations")
 
+    # The projector launches of one iteration in the launch shape the timed
+    # hipGraph uses (single-frame launches on the frame-group streams,
+    # replayed as a graph), and in the eager batched shape, HIP events around
+    # each (stt_time_projector; outputs to scratch).
+    proj_graph_ms = ctx.time_projector(1, reps=3) if use_graph else None
+    proj_eager_ms = ctx.time_projector(0, reps=3)
+    # Re-trace of the flows after a flow change (the command-line loop's
+    # cadence: once per outer iteration, i.e. every 2 PD passes in Render):
+    # rewrite every flow with its own values, then time the preparation.
+    for f in range(cfg["F"] - 1):
+        ctx.set_flow(f, ctx.get_buffer(stt.STT_BUF_FLOW, f))
+    tr = ctx.iterate(0, batched=batched, use_graph=False, timing=True)
+    log(f"projector phase: graph form {proj_graph_ms} ms, eager {proj_eager_ms:.3f} ms; "
+        f"re-trace of {tr.trace_launches} flows {tr.trace_ms:.3f} ms")
+
     vol, det, F = cfg["vol"], cfg["det"], cfg["F"]
     Ps = [views_of(cfg, f) for f in range(F)]
     V = int(np.prod(vol))
@@ -587,6 +602,7 @@ def main():
     achieved = fp_bytes / fp_avg_s
     iter_bytes = sum(Ps) * (12 * V + 12 * D) + F * 124 * V
     ms_per_step = elapsed / args.steps * 1e3
+    proj_iter_bytes = sum(Ps) * (4 * V + 8 * D)  # every frame-view's projector bytes
 
     views_desc = str(Ps[0]) if len(set(Ps)) == 1 else f"{min(Ps)}-{max(Ps)} (seeded, total {sum(Ps)})"
     traffic = load_traffic(args.config, tile) if batched else None
@@ -625,6 +641,27 @@ def main():
             "bytes_per_launch": fp_bytes,
         },
         "traffic_source": None if traffic is None else traffic["source"],
+        # the projector phase of one iteration in the timed form's own launch
+        # shape (single-frame launches on 8 frame-group streams, one hipGraph),
+        # and in the eager batched shape of `roofline`, both timed live
+        "roofline_graph_form": None if proj_graph_ms is None else {
+            "bound": "hbm",
+            "kernel": "projector launches of one iteration, hipGraph of single-frame launches on frame-group streams",
+            "achieved": proj_iter_bytes / (proj_graph_ms / 1e3) / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": proj_iter_bytes / (proj_graph_ms / 1e3) / HBM_PEAK,
+            "ms_per_iteration": proj_graph_ms,
+            "bytes_per_iteration": proj_iter_bytes,
+        },
+        "projector_eager_ms_per_iteration": proj_eager_ms,
+        # The flows stay fixed over the timed iterations, as over the PD passes
+        # of one outer iteration; after a flow change the PD warps' trace end
+        # points are recomputed once (k_trace).  At the reference's cadence of
+        # 2 PD passes per flow (ST_Tomography.cpp:477) that adds half a re-trace
+        # to every iteration:
+        "retrace_ms_per_flow_change": tr.trace_ms,
+        "value_retrace_every_2_passes": 1e3 / (ms_per_step + tr.trace_ms / 2),
         "kernel_ms_per_step": {
             "fp": fp_ms / ksteps, "bp": bp_ms / ksteps, "pd_dual": dual_ms / ksteps, "pd_primal": primal_ms / ksteps,
             "layout_copy": aux_ms / ksteps,

@@ -96,6 +96,10 @@ typedef struct stt_timing {
     double dual_ms;      int dual_launches;
     double primal_ms;    int primal_launches;
     double aux_ms;       int aux_launches;   /* layout copies feeding the projector */
+    /* Preparation before the iterations: re-tracing the flows written since
+     * the last PD step (trace_launches of them; once per flow change, not per
+     * iteration), plus any rebuild of the launch tables. */
+    double trace_ms;     int trace_launches;
 } stt_timing;
 
 typedef struct stt_ctx stt_ctx;
@@ -250,6 +254,15 @@ int  stt_set_projector_tile(stt_ctx* ctx, int rays_per_row);
  * the choice and ms_per_shape[5] (may be NULL) the time per view index of
  * shapes 0/64/32/16/8. */
 int  stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape);
+
+/* Measurement helper: the projector launches of one iteration (every
+ * frame's views, layout copies included, SART mode) timed with HIP events,
+ * averaged over reps.  graph_form = 0: the batched eager form, one launch per
+ * view index over all frames; 1: the launch shape stt_iterate's hipGraph
+ * uses (single-frame launches on the frame-group streams), captured and
+ * replayed.  Corrections and ray lengths go to scratch buffers: no solver
+ * state changes. */
+int  stt_time_projector(stt_ctx* ctx, int graph_form, int reps, double* ms_per_iteration);
 
 /* Blocks until the context stream is idle. */
 int  stt_synchronize(stt_ctx* ctx);

@@ -58,7 +58,8 @@ class Timing(C.Structure):
                 ("bp_ms", C.c_double), ("bp_launches", C.c_int),
                 ("dual_ms", C.c_double), ("dual_launches", C.c_int),
                 ("primal_ms", C.c_double), ("primal_launches", C.c_int),
-                ("aux_ms", C.c_double), ("aux_launches", C.c_int)]
+                ("aux_ms", C.c_double), ("aux_launches", C.c_int),
+                ("trace_ms", C.c_double), ("trace_launches", C.c_int)]
 
 
 class ImageInfo(C.Structure):
@@ -116,6 +117,7 @@ _SIGS = {
     "stt_pd_extrapolate": (C.c_int, [_P, C.c_int]),
     "stt_iterate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Timing)]),
     "stt_synchronize": (C.c_int, [_P]),
+    "stt_time_projector": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "stt_set_projector_tile": (C.c_int, [_P, C.c_int]),
     "stt_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "stt_residual_norm": (C.c_int, [_P, C.POINTER(C.c_double)]),
@@ -501,6 +503,12 @@ class Context:
 
     def synchronize(self):
         check(lib.stt_synchronize(self.h))
+
+    def time_projector(self, graph_form, reps=3):
+        """ms of one iteration's projector launches (stt_time_projector)."""
+        ms = C.c_double()
+        check(lib.stt_time_projector(self.h, int(graph_form), int(reps), C.byref(ms)))
+        return ms.value
 
     def set_projector_tile(self, rays_per_row):
         check(lib.stt_set_projector_tile(self.h, rays_per_row))

@@ -92,6 +92,12 @@ struct OwnedStream {
         if (s) (void)hipStreamDestroy(s);
     }
 };
+struct OwnedEvent {
+    hipEvent_t e = nullptr;
+    ~OwnedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
 struct OwnedGraphExec {
     hipGraphExec_t e = nullptr;
     ~OwnedGraphExec() {
@@ -753,7 +759,9 @@ struct stt_ctx {
     // interpolation (stream-ordered before the PD steps that read them).  The
     // first call allocates them, 24 bytes per voxel and frame; if the device
     // refuses that memory, the PD kernels keep tracing in place.
+    int last_retraced = 0;  // flows re-traced by the latest refresh_traces
     int refresh_traces() {
+        last_retraced = 0;
         if (!trace_tried) {
             trace_tried = true;
             std::vector<void*> got;
@@ -780,6 +788,7 @@ struct stt_ctx {
                 if (trace_gen[f] == flow_gen[f] && trace_interp == pc.interp) continue;
                 HIPCHK(launch_trace(flow[f], trace_bw[f], trace_fw[f], vg, pc.interp, stream));
                 trace_gen[f] = flow_gen[f];
+                ++last_retraced;
             }
             trace_interp = pc.interp;
         }
@@ -1396,14 +1405,20 @@ int stt_pd_extrapolate(stt_ctx* ctx, int frame) {
 int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timing* timing) {
     CTX_CHECK(ctx);
     if (n_iters < 0) return fail(STT_ERR_INVALID, "negative iteration count");
+    if (timing) std::memset(timing, 0, sizeof(*timing));
+    OwnedEvent p0, p1, e0, e1;
+    if (timing) {
+        HIPCHK(hipEventCreate(&p0.e));
+        HIPCHK(hipEventCreate(&p1.e));
+        HIPCHK(hipEventCreate(&e0.e));
+        HIPCHK(hipEventCreate(&e1.e));
+        HIPCHK(hipEventRecord(p0.e, ctx->stream));
+    }
     int rc = ctx->prepare_iteration();
     if (rc) return rc;
-    if (timing) std::memset(timing, 0, sizeof(*timing));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) {
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
+        HIPCHK(hipEventRecord(p1.e, ctx->stream));
+        HIPCHK(hipEventRecord(e0.e, ctx->stream));
     }
     if (use_graph) {
         if (!ctx->graph_exec || ctx->graph_batched != (batched ? 1 : 0)) {
@@ -1433,14 +1448,105 @@ int stt_iterate(stt_ctx* ctx, int n_iters, int batched, int use_graph, stt_timin
         }
     }
     if (timing) {
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventRecord(e1.e, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1.e));
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0.e, e1.e));
         timing->total_ms = ms;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        HIPCHK(hipEventElapsedTime(&ms, p0.e, p1.e));
+        timing->trace_ms = ms;
+        timing->trace_launches = ctx->last_retraced;
     }
+    return STT_OK;
+}
+
+int stt_time_projector(stt_ctx* ctx, int graph_form, int reps, double* ms_per_iteration) {
+    CTX_CHECK(ctx);
+    if (!ms_per_iteration || reps < 1) return fail(STT_ERR_INVALID, "need reps >= 1 and an output");
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    if ((rc = ctx->build_tables())) return rc;
+    const size_t D = ctx->D;
+    // every frame's views in the form's launch tables, with the corrections
+    // and ray lengths redirected to scratch (no solver state changes)
+    std::vector<WorkItem> items;
+    std::vector<std::array<int, 3>> launches;  // (group, first item, count)
+    if (graph_form) {
+        for (int h = 0; h < ctx->ngroups; ++h)
+            for (int k = 0; k < ctx->split_maxP[h]; ++k) {
+                launches.push_back({h, (int)items.size(), ctx->split_cnt[h][k]});
+                for (int f = ctx->split_f[h]; f < ctx->split_f[h + 1]; ++f)
+                    if (k < ctx->P[f]) items.push_back(ctx->make_item(f, k, false));
+            }
+    } else {
+        for (int k = 0; k < ctx->maxP; ++k) {
+            launches.push_back({0, (int)items.size(), ctx->batch_cnt[k]});
+            for (int f = 0; f < ctx->F; ++f)
+                if (k < ctx->P[f]) items.push_back(ctx->make_item(f, k, false));
+        }
+    }
+    StreamMem buf_mem, tab_mem;
+    HIPCHK(buf_mem.alloc(2 * items.size() * D * sizeof(float), ctx->stream));
+    HIPCHK(tab_mem.alloc(items.size() * sizeof(WorkItem), ctx->stream));
+    auto* buf = (float*)buf_mem.p;
+    auto* tab = (WorkItem*)tab_mem.p;
+    for (size_t n = 0; n < items.size(); ++n) {
+        items[n].corr = buf + 2 * n * D;
+        items[n].raylen = buf + (2 * n + 1) * D;
+        items[n].last = 0;
+    }
+    HIPCHK(hipMemsetAsync(buf, 0, 2 * items.size() * D * sizeof(float), ctx->stream));
+    HIPCHK(hipMemcpyAsync(tab, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    auto issue = [&](bool multi) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (multi) {
+            e = hipEventRecord(ctx->ev_fork, ctx->stream);
+            for (int h = 1; h < ctx->ngroups && e == hipSuccess; ++h) e = hipStreamWaitEvent(ctx->streams[h], ctx->ev_fork, 0);
+        }
+        for (const auto& l : launches) {
+            hipStream_t s = multi ? ctx->streams[l[0]] : ctx->stream;
+            if (e == hipSuccess) e = launch_transpose_xz(tab + l[1], l[2], ctx->vg, s);
+            if (e == hipSuccess) e = launch_fp(FP_MODE_SART, tab + l[1], l[2], ctx->vg, ctx->pc.factor, s);
+        }
+        if (multi)
+            for (int h = 1; h < ctx->ngroups && e == hipSuccess; ++h) {
+                e = hipEventRecord(ctx->ev_join[h], ctx->streams[h]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->ev_join[h], 0);
+            }
+        return e;
+    };
+    OwnedGraphExec exec;
+    if (graph_form) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t ie = issue(ctx->ngroups >= 2);
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (ie != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(STT_ERR_HIP, std::string("projector graph capture: ") +
+                                         hipGetErrorString(ie != hipSuccess ? ie : ce));
+        }
+        const hipError_t ge = hipGraphInstantiate(&exec.e, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ge);
+        HIPCHK(hipGraphLaunch(exec.e, ctx->stream));  // warm-up
+    } else {
+        HIPCHK(issue(false));  // warm-up
+    }
+    OwnedEvent t0, t1;
+    HIPCHK(hipEventCreate(&t0.e));
+    HIPCHK(hipEventCreate(&t1.e));
+    HIPCHK(hipEventRecord(t0.e, ctx->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (graph_form) HIPCHK(hipGraphLaunch(exec.e, ctx->stream));
+        else HIPCHK(issue(false));
+    }
+    HIPCHK(hipEventRecord(t1.e, ctx->stream));
+    HIPCHK(hipEventSynchronize(t1.e));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t0.e, t1.e));
+    *ms_per_iteration = ms / reps;
     return STT_OK;
 }
 
