@@ -129,6 +129,16 @@ int  stt_set_views(stt_ctx* ctx, int frame, const float* angles_deg, int n);
 /* Explicit rotations, x,y,z,w per view (pQuaternion[frame][view]). */
 int  stt_set_view_quaternions(stt_ctx* ctx, int frame, const float* q_xyzw, int n);
 
+/* Explicit camera per view, as TypicalRayProducer::SetViewMatrix /
+ * SetProjectionMatrix take it (TypicalRayProducer.h:22-23): column-major 4x4
+ * modelview (View = T * S * R, Scene.cpp:181-207) and projection (frustum,
+ * Scene.cpp:69-138) matrices, n of each; the per-view record then follows
+ * TypicalRayProducer::_calculateMatrix (TypicalRayProducer.cpp:142-208). */
+int  stt_set_view_matrices(stt_ctx* ctx, int frame, const float* view16, const float* proj16, int n);
+/* The Scene's modelview and projection matrices for a rotation (x,y,z,w) in
+ * this geometry (camera of ST_Tomography.cpp:229-246).  Host only. */
+int  stt_scene_matrices(const stt_geometry* g, const float* q_xyzw, float* view16, float* proj16);
+
 /* Data (VolumeData::m_Volumelist / m_Flowlist, Projslist). frame in [0, nframes];
  * frame == nframes is the padding copy. */
 int  stt_set_volume(stt_ctx* ctx, int frame, const float* vol);

@@ -95,6 +95,8 @@ _SIGS = {
     "stt_ctx_get_gmax": (C.c_int, [_P, _F]),
     "stt_set_views": (C.c_int, [_P, C.c_int, _F, C.c_int]),
     "stt_set_view_quaternions": (C.c_int, [_P, C.c_int, _F, C.c_int]),
+    "stt_set_view_matrices": (C.c_int, [_P, C.c_int, _F, _F, C.c_int]),
+    "stt_scene_matrices": (C.c_int, [C.POINTER(Geometry), _F, _F, _F]),
     "stt_set_volume": (C.c_int, [_P, C.c_int, _F]),
     "stt_get_volume": (C.c_int, [_P, C.c_int, _F]),
     "stt_set_flow": (C.c_int, [_P, C.c_int, _F]),

@@ -1046,6 +1046,30 @@ int stt_set_view_quaternions(stt_ctx* ctx, int frame, const float* q, int n) {
     return STT_OK;
 }
 
+int stt_set_view_matrices(stt_ctx* ctx, int frame, const float* view16, const float* proj16, int n) {
+    CTX_CHECK(ctx);
+    if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
+    if (!view16 || !proj16 || n != ctx->P[frame])
+        return fail(STT_ERR_INVALID, "need n = views_per_frame[frame] view and projection matrices");
+    ViewRec* dst = ctx->views_host.data() + ctx->view_off[frame];
+    for (int i = 0; i < n; ++i) dst[i] = build_view_from(ctx->g, view16 + 16 * i, proj16 + 16 * i);
+    HIPCHK(hipMemcpyAsync(ctx->views_dev + ctx->view_off[frame], dst, n * sizeof(ViewRec), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->views_set[frame] = 1;
+    ctx->tables_dirty = true;
+    ctx->invalidate_graph();
+    return STT_OK;
+}
+
+int stt_scene_matrices(const stt_geometry* g, const float* q_xyzw, float* view16, float* proj16) {
+    int rc = valid_geometry(g);
+    if (rc) return rc;
+    if (!q_xyzw || !view16 || !proj16) return fail(STT_ERR_INVALID, "null argument");
+    scene_matrices(*g, q_xyzw, view16, proj16);
+    return STT_OK;
+}
+
 int stt_set_views(stt_ctx* ctx, int frame, const float* angles_deg, int n) {
     CTX_CHECK(ctx);
     if (!angles_deg) return fail(STT_ERR_INVALID, "null angles");

@@ -149,9 +149,9 @@ void quat_rotate_y(const float q_old[4], double deg, float q[4]) {
     q[2] = pw * qz + pz * qw + px * qy - py * qx;
 }
 
-ViewRec build_view(const stt_geometry& g, const float q[4]) {
-    // Camera of the ST_Tomography ctor and Scene::_calculateCamera
-    // (ST_Tomography.cpp:229-246; Scene.cpp:77,131,133).
+void scene_matrices(const stt_geometry& g, const float q[4], float view_out[16], float proj_out[16]) {
+    // Camera of the ST_Tomography ctor and Scene::_calculateCamera /
+    // _calculateMatrix (ST_Tomography.cpp:229-246; Scene.cpp:77,131,133,181-207).
     const double halfW = (g.det[0] - 1) * g.ds * 0.5f;
     const double halfH = (g.det[1] - 1) * g.ds * 0.5f;
     const float sid = (float)g.sid, sdd = (float)g.sdd;
@@ -168,6 +168,20 @@ ViewRec build_view(const stt_geometry& g, const float q[4]) {
     proj.e[10] = -(f + n) / (f - n);
     proj.e[11] = -1.0f;
     proj.e[14] = -2.0f * f * n / (f - n);
+    std::memcpy(view_out, view.e, sizeof(view.e));
+    std::memcpy(proj_out, proj.e, sizeof(proj.e));
+}
+
+ViewRec build_view_from(const stt_geometry& g, const float view16[16], const float proj16[16]) {
+    // TypicalRayProducer::_calculateMatrix (TypicalRayProducer.cpp:142-208)
+    // from the modelview and projection matrices the solver sets
+    // (SetViewMatrix / SetProjectionMatrix).
+    const double halfW = (g.det[0] - 1) * g.ds * 0.5f;
+    const double halfH = (g.det[1] - 1) * g.ds * 0.5f;
+    const float sdd = (float)g.sdd;
+    Mat4 view, proj;
+    std::memcpy(view.e, view16, sizeof(view.e));
+    std::memcpy(proj.e, proj16, sizeof(proj.e));
 
     float half_extent[3];
     for (int c = 0; c < 3; ++c) half_extent[c] = -((g.vol[c] - 1) * g.spacing[c]) * 0.5f;
@@ -225,6 +239,12 @@ ViewRec build_view(const stt_geometry& g, const float q[4]) {
     rec.sdd = sdd;
     rec.ds = g.ds;
     return rec;
+}
+
+ViewRec build_view(const stt_geometry& g, const float q[4]) {
+    float view[16], proj[16];
+    scene_matrices(g, q, view, proj);
+    return build_view_from(g, view, proj);
 }
 
 float sample_distance(const stt_geometry& g) {
