@@ -226,6 +226,13 @@ int  stt_pd_dual(stt_ctx* ctx, int frame);
 int  stt_pd_primal(stt_ctx* ctx, int frame);
 /* X_bar = 2x - X_prev on the ROI (ST_Tomography.cpp:586-604). */
 int  stt_pd_extrapolate(stt_ctx* ctx, int frame);
+/* Memory knob with no counterpart in the reference (which re-traces in every
+ * warp, warping.h:100-114): keep = 1 (the default) stores the PD traces' end
+ * points, 24 bytes per voxel and frame, allocated on the next PD step;
+ * keep = 0 frees them and the PD kernels trace in place (the same float end
+ * points, so bit-identical results, at the cost of the velocity sampling in
+ * every PD step).  At 512^3 x 16 frames the stored traces take 51.5 GB. */
+int  stt_set_trace_storage(stt_ctx* ctx, int keep);
 
 /* One or more ST iterations (one "No_PO" pass over all frames,
  * ST_Tomography.cpp:477-608): per frame dual, primal, SART sweep over all of

@@ -117,6 +117,7 @@ _SIGS = {
     "stt_pd_dual": (C.c_int, [_P, C.c_int]),
     "stt_pd_primal": (C.c_int, [_P, C.c_int]),
     "stt_pd_extrapolate": (C.c_int, [_P, C.c_int]),
+    "stt_set_trace_storage": (C.c_int, [_P, C.c_int]),
     "stt_iterate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Timing)]),
     "stt_synchronize": (C.c_int, [_P]),
     "stt_time_projector": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -497,6 +498,10 @@ class Context:
 
     def pd_extrapolate(self, frame):
         check(lib.stt_pd_extrapolate(self.h, frame))
+
+    def set_trace_storage(self, keep):
+        """Keep (True) or free (False) the stored PD trace end points."""
+        check(lib.stt_set_trace_storage(self.h, int(bool(keep))))
 
     def iterate(self, n=1, batched=True, use_graph=False, timing=False):
         t = Timing() if timing else None

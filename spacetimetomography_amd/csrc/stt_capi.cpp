@@ -447,6 +447,7 @@ struct stt_ctx {
     std::vector<uint64_t> flow_gen, trace_gen;
     int trace_interp = -1;
     bool trace_tried = false;
+    bool keep_traces = true;  // stt_set_trace_storage
     ViewRec* views_dev = nullptr;
     std::vector<ViewRec> views_host;
     std::vector<char> views_set;
@@ -598,6 +599,7 @@ struct stt_ctx {
             fbs[f].trace_bw = f < (int)trace_bw.size() ? trace_bw[f] : nullptr;
             fbs[f].trace_fw = f < (int)trace_fw.size() ? trace_fw[f] : nullptr;
         }
+        pc.stored_traces = (int)trace_bw.size() == F && (int)trace_fw.size() == F;
         HIPCHK(hipMemcpyAsync(fb_dev, fbs.data(), fbs.size() * sizeof(FrameBufs), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         tables_dirty = false;
@@ -678,7 +680,9 @@ struct stt_ctx {
             // and primals of all frames run first, as one launch each (a
             // group's primal run beside the other groups' sweeps made the
             // iteration 0.7% slower, and no faster once the primal read
-            // stored trace end points).  Joined before returning.
+            // stored trace end points; per-group duals and primals, each
+            // group's first primal waiting for the previous group's last
+            // dual, were 0.5% slower).  Joined before returning.
             HIPCHK(launch_pd_dual(fb_dev, 0, F, vg, pc, stream));
             HIPCHK(launch_pd_primal(fb_dev, 0, F, vg, pc, stream));
             HIPCHK(hipEventRecord(ev_fork, stream));
@@ -762,7 +766,7 @@ struct stt_ctx {
     int last_retraced = 0;  // flows re-traced by the latest refresh_traces
     int refresh_traces() {
         last_retraced = 0;
-        if (!trace_tried) {
+        if (!trace_tried && keep_traces) {
             trace_tried = true;
             std::vector<void*> got;
             bool ok = true;
@@ -778,6 +782,7 @@ struct stt_ctx {
                 }
                 allocs.insert(allocs.end(), got.begin(), got.end());
                 tables_dirty = true;
+                invalidate_graph();  // the PD launches switch to the stored-trace kernels
             } else {
                 for (void* a : got) (void)hipFree(a);  // none of it is kept
                 (void)hipGetLastError();              // the refused allocation is not an error of the next call
@@ -793,6 +798,23 @@ struct stt_ctx {
             trace_interp = pc.interp;
         }
         return build_tables();
+    }
+
+    // Release the stored trace end points (stt_set_trace_storage(ctx, 0)).
+    int drop_traces() {
+        if (trace_bw.empty()) return STT_OK;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int f = 0; f < F; ++f)
+            for (float* a : {trace_bw[f], trace_fw[f]}) {
+                allocs.erase(std::find(allocs.begin(), allocs.end(), (void*)a));
+                HIPCHK(hipFree(a));
+            }
+        trace_bw.clear();
+        trace_fw.clear();
+        trace_gen.assign(F, 0);
+        tables_dirty = true;
+        invalidate_graph();
+        return STT_OK;
     }
 
     void invalidate_graph() {
@@ -1394,6 +1416,17 @@ int stt_pd_init(stt_ctx* ctx) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
+}
+
+int stt_set_trace_storage(stt_ctx* ctx, int keep) {
+    CTX_CHECK(ctx);
+    if (keep != 0 && keep != 1) return fail(STT_ERR_INVALID, "keep must be 0 or 1");
+    ctx->keep_traces = keep == 1;
+    if (keep) {
+        ctx->trace_tried = !ctx->trace_bw.empty();  // a refused allocation is tried again
+        return STT_OK;
+    }
+    return ctx->drop_traces();
 }
 
 int stt_pd_dual(stt_ctx* ctx, int frame) {

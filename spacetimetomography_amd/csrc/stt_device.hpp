@@ -58,6 +58,7 @@ struct PdConst {
     int prior;
     int interp;
     int nframes;
+    int stored_traces;  // 1: FrameBufs::trace_bw/trace_fw hold every frame's PD trace end points
 };
 
 // One frame's share of a projector / backprojector launch.

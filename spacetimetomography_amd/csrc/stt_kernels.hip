@@ -1183,6 +1183,7 @@ __global__ __launch_bounds__(256) void k_transpose_xz(const WorkItem* __restrict
 // Catmull-Rom weights of cubicInterpolate (ST_Tomography.cpp:183-186),
 // identical polynomial to CImg's cubic (CImg.h:13429-13520).
 __device__ __forceinline__ void cr_weights(float t, float w[4]) {
+#pragma clang fp contract(off)
     const float t2 = t * t, t3 = t2 * t;
     w[0] = 0.5f * (-t + 2.0f * t2 - t3);
     w[1] = 1.0f + 0.5f * (-5.0f * t2 + 3.0f * t3);
@@ -1564,11 +1565,13 @@ __device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned 
             for (int ky = 0; ky < 2; ++ky)
                 r[kz][ky] = __builtin_bit_cast(
                     float2, __builtin_amdgcn_raw_buffer_load_b64(rs, s.xoff + s.yoff[ky] + s.zoff[kz], plane_off, 0));
+        // explicit fmaf: the same rounding wherever this is inlined (the
+        // stored trace end points and the in-place traces must agree)
         const float gx = 1.0f - s.fx, gy = 1.0f - s.fy;
-        const float c00 = r[0][0].x * gx + r[0][0].y * s.fx, c10 = r[0][1].x * gx + r[0][1].y * s.fx;
-        const float c01 = r[1][0].x * gx + r[1][0].y * s.fx, c11 = r[1][1].x * gx + r[1][1].y * s.fx;
-        const float e0 = c00 * gy + c10 * s.fy, e1 = c01 * gy + c11 * s.fy;
-        return e0 * (1.0f - s.fz) + e1 * s.fz;
+        const float c00 = fmaf(r[0][0].y, s.fx, r[0][0].x * gx), c10 = fmaf(r[0][1].y, s.fx, r[0][1].x * gx);
+        const float c01 = fmaf(r[1][0].y, s.fx, r[1][0].x * gx), c11 = fmaf(r[1][1].y, s.fx, r[1][1].x * gx);
+        const float e0 = fmaf(c10, s.fy, c00 * gy), e1 = fmaf(c11, s.fy, c01 * gy);
+        return fmaf(e1, s.fz, e0 * (1.0f - s.fz));
     }
     float mn = INFINITY, mx = -INFINITY;
     float acc_z = 0.0f;
@@ -1584,7 +1587,7 @@ __device__ __forceinline__ float sample_img(__amdgpu_buffer_rsrc_t rs, unsigned 
                 mn = fminf(mn, fminf(row.y, row.z));
                 mx = fmaxf(mx, fmaxf(row.y, row.z));
             }
-            const float rv = s.wx[0] * row.x + s.wx[1] * row.y + s.wx[2] * row.z + s.wx[3] * row.w;
+            const float rv = fmaf(s.wx[3], row.w, fmaf(s.wx[2], row.z, fmaf(s.wx[1], row.y, s.wx[0] * row.x)));
             acc_y = fmaf(s.wy[ky], rv, acc_y);
         }
         acc_z = fmaf(s.wz[kz], acc_y, acc_z);
@@ -1644,17 +1647,25 @@ __device__ __forceinline__ float warp_voxel(const float* img, const float* uvw, 
 
 // The PD steps' warp of img at voxel (i,j,k): from the stored end points of
 // the trace when the context keeps them (three planes of V floats), else
-// traced here.  Both give the same float end point, so the same sample.
-__device__ __forceinline__ float pd_warp(const float* img, const float* uvw, const float* tr, const VolGeom& g,
-                                         int interp, int i, int j, int k, size_t qi, float tdt) {
-    if (tr) {
+// traced here.  Both give the same float end point, so the same sample.  The
+// form is a template argument (pd_warp_kind), so each PD kernel instance
+// carries one sampling path and the registers of that path only.
+template <int WK>
+__device__ __forceinline__ float pd_warp(const float* img, const float* uvw, const float* tr, const VolGeom& g, int i,
+                                         int j, int k, size_t qi, float tdt) {
+    constexpr bool kCubic = (WK & 1) == 0;
+    if (WK < 2) {
         const size_t V = (size_t)g.nx * g.ny * g.nz;
         const gfloat* t = gptr(tr);
         const float q[3] = {t[qi], t[V + qi], t[2 * V + qi]};
-        return interp == 0 ? sample_at<true>(img, g, q) : sample_at<false>(img, g, q);
+        return sample_at<kCubic>(img, g, q);
     }
-    return interp == 0 ? warp_voxel<true>(img, uvw, g, i, j, k, tdt) : warp_voxel<false>(img, uvw, g, i, j, k, tdt);
+    return warp_voxel<kCubic>(img, uvw, g, i, j, k, tdt);
 }
+
+// 0: stored end points, clamped cubic; 1: stored, trilinear; 2: traced here,
+// clamped cubic; 3: traced here, trilinear.
+inline int pd_warp_kind(const PdConst& pc) { return (pc.stored_traces ? 0 : 2) + (pc.interp == 0 ? 0 : 1); }
 
 // End points of the two PD traces of flow u_f at every voxel: step +1 (the
 // dual's backward warp, warp(u_f, ., dt = -1)) into tb, step -1 (the primal's
@@ -1692,6 +1703,7 @@ __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ uvw, con
 }
 
 // Dual step of frame f (ST_Tomography.cpp:502-534); grid.z = z + nz*(f - f0).
+template <int WK>
 __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ fb, int f0, VolGeom g, PdConst pc) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1709,7 +1721,7 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
     gfloat* Ytp = gptr(B.ytp);
     gfloat* Ybc = gptr(B.ybc);
     // m_backward_warp = warp(u_f, X_bar[f+1], dt = -1): trace step +1 (:503)
-    const float bw = pd_warp(B1.xbar, B.flow, B.trace_bw, g, pc.interp, x, y, z, q, 1.0f);
+    const float bw = pd_warp<WK>(B1.xbar, B.flow, B.trace_bw, g, x, y, z, q, 1.0f);
     const float xq = xb0[q];
     float ytp = Ytp[q] + pc.sigma * (xb1[q] - xq);  // (:504)
     float ybc = Ybc[q] + pc.sigma * (bw - xq);      // (:505)
@@ -1744,6 +1756,7 @@ __global__ __launch_bounds__(256) void k_pd_dual(const FrameBufs* __restrict__ f
 }
 
 // Primal step of frame f (ST_Tomography.cpp:536-578).
+template <int WK>
 __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__ fb, int f0, VolGeom g, PdConst pc) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1764,7 +1777,7 @@ __global__ __launch_bounds__(256) void k_pd_primal(const FrameBufs* __restrict__
         xv = xv - pc.tau * (-Ybc[q]);
     } else {                // (:546-555): forward warp of Y_bc[f-1] with u_{f-1}, trace step -1
         const FrameBufs& Bm = fb[f - 1];
-        const float fw = pd_warp(Bm.ybc, Bm.flow, Bm.trace_fw, g, pc.interp, x, y, z, q, -1.0f);
+        const float fw = pd_warp<WK>(Bm.ybc, Bm.flow, Bm.trace_fw, g, x, y, z, q, -1.0f);
         xv = xv - pc.tau * (gptr(Bm.ytp)[q] - Ytp[q]);
         xv = xv - pc.tau * (fw - Ybc[q]);
     }
@@ -2475,12 +2488,24 @@ hipError_t launch_bp_ray_apply(const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_pd_dual(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s) {
-    hipLaunchKernelGGL(k_pd_dual, vol_grid(g, nf), dim3(256), 0, s, fb, f0, g, pc);
+    const dim3 grid = vol_grid(g, nf);
+    switch (pd_warp_kind(pc)) {
+        case 0: hipLaunchKernelGGL(k_pd_dual<0>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+        case 1: hipLaunchKernelGGL(k_pd_dual<1>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+        case 2: hipLaunchKernelGGL(k_pd_dual<2>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+        default: hipLaunchKernelGGL(k_pd_dual<3>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_pd_primal(const FrameBufs* fb, int f0, int nf, const VolGeom& g, const PdConst& pc, hipStream_t s) {
-    hipLaunchKernelGGL(k_pd_primal, vol_grid(g, nf), dim3(256), 0, s, fb, f0, g, pc);
+    const dim3 grid = vol_grid(g, nf);
+    switch (pd_warp_kind(pc)) {
+        case 0: hipLaunchKernelGGL(k_pd_primal<0>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+        case 1: hipLaunchKernelGGL(k_pd_primal<1>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+        case 2: hipLaunchKernelGGL(k_pd_primal<2>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+        default: hipLaunchKernelGGL(k_pd_primal<3>, grid, dim3(256), 0, s, fb, f0, g, pc); break;
+    }
     return hipGetLastError();
 }
 

@@ -415,6 +415,36 @@ def test_interp_change_recomputes_traces():
     ctx.close()
 
 
+@pytest.mark.parametrize("interp", [0, 1])
+def test_trace_storage_off_is_bit_identical(interp):
+    """The PD kernels that trace in place (stt_set_trace_storage(ctx, 0)) give
+    the stored-trace kernels' results bit for bit, in every form of the
+    iteration, and storage switched off and on again mid-run changes nothing."""
+    print(f"[trace-storage] F=3 24x22x26, interp {interp}, STV", flush=True)
+    p = stt.params_default()
+    p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
+    p.interp, p.prior = interp, stt.STT_PRIOR_STV
+    vol, P = (24, 22, 26), (4, 4, 4)
+    ctxs = []
+    for _ in range(2):
+        ctxs.append(_setup_pair(vol, (32, 32), P, lambda f: [0, 90, 180, 270], p, F=3)[1])
+    kept, traced = ctxs
+    traced.set_trace_storage(False)
+    for batched, graph in ((False, False), (True, False), (True, True)):
+        kept.iterate(1, batched=batched, use_graph=graph)
+        traced.iterate(1, batched=batched, use_graph=graph)
+    traced.set_trace_storage(True)
+    kept.set_trace_storage(False)
+    for c in ctxs:
+        c.iterate(1, batched=True, use_graph=True)
+    for f in range(3):
+        assert np.array_equal(kept.get_volume(f), traced.get_volume(f)), f
+        for which in (stt.STT_BUF_YBC, stt.STT_BUF_YTV, stt.STT_BUF_XBAR):
+            assert np.array_equal(kept.get_buffer(which, f), traced.get_buffer(which, f)), (f, which)
+    for c in ctxs:
+        c.close()
+
+
 def test_batched_equals_plain():
     print("[batched-vs-plain] F=4", flush=True)
     p = stt.params_default()
