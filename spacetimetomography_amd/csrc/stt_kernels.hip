@@ -717,6 +717,7 @@ struct SlabBox {
     int px, pxy;       // LDS row and plane pitch (odd, so lanes on one x spread over the banks)
     int base;          // LDS index of (lo), subtracted from tap indices
     bool staged;
+    bool inner;        // every sample of the slab lies inside [0, dims-1]^3: no clamp needed
 };
 
 // A slab box passes from the wave that computes it to the whole block
@@ -725,7 +726,7 @@ constexpr int kBoxInts = 14;
 __device__ __forceinline__ void box_put(int* slot, const SlabBox& b) {
     if ((threadIdx.x & 63) == 0) {
         const int v[kBoxInts] = {b.lo[0], b.lo[1], b.lo[2], b.n[0], b.n[1], b.n[2], b.cbeg,
-                                 b.cend,  b.px,    b.pxy,   b.base, (int)b.staged, 0, 0};
+                                 b.cend,  b.px,    b.pxy,   b.base, (int)b.staged, (int)b.inner, 0};
 #pragma unroll
         for (int k = 0; k < kBoxInts; ++k) slot[k] = v[k];
     }
@@ -745,6 +746,7 @@ __device__ __forceinline__ SlabBox box_get(const int* slot) {
     b.pxy = v[9];
     b.base = v[10];
     b.staged = v[11] != 0;
+    b.inner = v[12] != 0;
     return b;
 }
 
@@ -846,6 +848,16 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap &&
                 bx.px * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
     bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
+    // Inner slab: the sampler's clamp to [0, dims-1] is the identity on all
+    // of its samples.  Transversally the samples lie within eps of the
+    // extremes a0..a1, b0..b1.  Along D a sample belongs to the slab when its
+    // clamped coordinate c lies in [c0, c0+K) (up) or [c0-K+1, c0+1) (down);
+    // with that range inside (0, bD) the clamp cannot have moved it (a
+    // position below 0 clamps to 0, one above bD to bD, both outside the
+    // range), so c is the position itself.
+    const float bD = D == 0 ? g.bx : (D == 1 ? g.by : g.bz);
+    bx.inner = bx.staged && a0 - eps >= 0.0f && a1 + eps <= bA && b0 - eps >= 0.0f && b1 + eps <= bB &&
+               (DOWN ? (c0 - K >= 0 && (float)(c0 + 1) <= bD) : (c0 >= 1 && (float)(c0 + K) <= bD));
     return bx;
 }
 
@@ -893,6 +905,57 @@ __device__ __forceinline__ void slab_stage(const VolView& vv, const VolGeom& g, 
 // Waits for the direct-to-LDS loads of slab_stage (vmcnt(0); expcnt and
 // lgkmcnt left at their maxima).
 __device__ __forceinline__ void slab_stage_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Samples of one slab from its staged box, without a per-sample box test
+// (every sample of the wave's rays lies in the box).  A lane leaves the loop
+// at its first sample past the slab, or at max_samples, or when its ray ends
+// (nlim = -1).  Without COUNT, an ended ray's lane gets an unreachable plane
+// limit instead (the loop still ends: every sample advances t towards a
+// finite segment end).  CLAMP = false on inner slabs, where the clamp to
+// [0, dims-1] is the identity (SlabBox::inner).
+template <int D, bool DOWN, bool COUNT, bool CLAMP>
+__device__ __forceinline__ void slab_walk(RayGen& G, const VolGeom& g, const SlabBox& bx, const float* box, float lim,
+                                          float& nlim, float& llim, float& nf, float& density) {
+    constexpr float kEnded = DOWN ? INFINITY : -INFINITY;
+    const char* bb = (const char*)(box - bx.base);
+    const unsigned px4 = 4u * (unsigned)bx.px, pxy4 = 4u * (unsigned)bx.pxy;
+    const float sdw = g.sd;
+#pragma unroll 1
+    for (;;) {
+        const float cx = CLAMP ? __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx) : G.pend[0];
+        const float cy = CLAMP ? __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by) : G.pend[1];
+        const float cz = CLAMP ? __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz) : G.pend[2];
+        const float cc[3] = {cx, cy, cz};
+        if (COUNT) {
+            if (!((DOWN ? cc[D] >= lim : cc[D] < lim) && nf < nlim)) break;
+        } else if (!(DOWN ? cc[D] >= llim : cc[D] < llim)) {
+            break;
+        }
+        const unsigned off =
+            ((unsigned)(int)cx << 2) + __umul24((unsigned)(int)cy, px4) + __umul24((unsigned)(int)cz, pxy4);
+        const float* b0 = (const float*)(bb + off);
+        Taps T;
+        T.fx = __builtin_amdgcn_fractf(cx);
+        T.fy = __builtin_amdgcn_fractf(cy);
+        T.fz = __builtin_amdgcn_fractf(cz);
+        T.r00 = make_float2(b0[0], b0[1]);
+        T.r10 = make_float2(b0[bx.px], b0[bx.px + 1]);
+        T.r01 = make_float2(b0[bx.pxy], b0[bx.pxy + 1]);
+        T.r11 = make_float2(b0[bx.pxy + bx.px], b0[bx.pxy + bx.px + 1]);
+        if (COUNT) nf += 1.0f;
+        G.t += sdw;
+        G.pend[0] += G.step[0];
+        G.pend[1] += G.step[1];
+        G.pend[2] += G.step[2];
+        if (__builtin_expect(!(G.t < G.tb), 0)) {
+            if (!raygen_next_segment(G, sdw)) {
+                nlim = -1.0f;
+                llim = kEnded;
+            }
+        }
+        density = fmaf(lerp_taps(T), sdw, density);
+    }
+}
 
 // COUNT: the sample count is an output (FP mode); the SART pass needs only
 // the line integral, so its unchecked loop keeps no count.
@@ -999,52 +1062,13 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         if (bx.staged && wave_in_box) {
             // Samples of the slab are those whose clamped base plane p = (int)cD
             // satisfies p < cend (up) or p > cend (down); cD >= 0, so the test
-            // runs on the float.  A lane leaves the loop at its first sample
-            // past the slab, or at max_samples, or when its ray ends (nlim = -1).
-            // Without COUNT, an ended ray's lane gets an unreachable plane
-            // limit instead (the loop still ends: every sample advances t
-            // towards a finite segment end).
+            // runs on the float.
             const float lim = DOWN ? (float)(bx.cend + 1) : (float)bx.cend;
             constexpr float kEnded = DOWN ? INFINITY : -INFINITY;
-            const char* bb = (const char*)(box - bx.base);
-            const unsigned px4 = 4u * (unsigned)bx.px, pxy4 = 4u * (unsigned)bx.pxy;
             float nlim = have ? maxn : -1.0f;
             float llim = have ? lim : kEnded;
-#pragma unroll 1
-            for (;;) {
-                const float cx = __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx);
-                const float cy = __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by);
-                const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
-                const float cc[3] = {cx, cy, cz};
-                if (COUNT) {
-                    if (!((DOWN ? cc[D] >= lim : cc[D] < lim) && nf < nlim)) break;
-                } else if (!(DOWN ? cc[D] >= llim : cc[D] < llim)) {
-                    break;
-                }
-                const unsigned off = ((unsigned)(int)cx << 2) + __umul24((unsigned)(int)cy, px4) +
-                                     __umul24((unsigned)(int)cz, pxy4);
-                const float* b0 = (const float*)(bb + off);
-                Taps T;
-                T.fx = __builtin_amdgcn_fractf(cx);
-                T.fy = __builtin_amdgcn_fractf(cy);
-                T.fz = __builtin_amdgcn_fractf(cz);
-                T.r00 = make_float2(b0[0], b0[1]);
-                T.r10 = make_float2(b0[bx.px], b0[bx.px + 1]);
-                T.r01 = make_float2(b0[bx.pxy], b0[bx.pxy + 1]);
-                T.r11 = make_float2(b0[bx.pxy + bx.px], b0[bx.pxy + bx.px + 1]);
-                if (COUNT) nf += 1.0f;
-                G.t += sdw;
-                G.pend[0] += G.step[0];
-                G.pend[1] += G.step[1];
-                G.pend[2] += G.step[2];
-                if (__builtin_expect(!(G.t < G.tb), 0)) {
-                    if (!raygen_next_segment(G, sdw)) {
-                        nlim = -1.0f;
-                        llim = kEnded;
-                    }
-                }
-                density = fmaf(lerp_taps(T), sdw, density);
-            }
+            if (bx.inner) slab_walk<D, DOWN, COUNT, false>(G, g, bx, box, lim, nlim, llim, nf, density);
+            else slab_walk<D, DOWN, COUNT, true>(G, g, bx, box, lim, nlim, llim, nf, density);
             have = COUNT ? nf < nlim : llim != kEnded;
             c0 = bx.cend;
             if (!__syncthreads_or(have)) break;

@@ -14,6 +14,11 @@
 // extremes widened by box_eps, clipped to [0, dims] at one end only);
 // bound = 1: the extremes clamped to [0, dims-1] before the floor, the
 // range the clamped sample positions actually take.
+//
+// Inner slabs (SlabBox::inner, walked without the clamp to [0, dims-1]): for
+// every sample of such a slab the clamped and unclamped positions must be
+// equal, and the unclamped slab-membership test must take the same samples
+// and stop at the same one; the model counts the samples where either fails.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -144,7 +149,7 @@ bool gen_init(Gen& G, const Ray& r, const Geo& g, bool hit, bool& entry_alone) {
 
 struct Box {
     int lo[3], n[3], cbeg, cend;
-    bool staged;
+    bool staged, inner;
 };
 
 constexpr int kSlabCap = 6144, kSlabK = 23, kSlabKMin = 8, kStagePasses = 4;
@@ -209,6 +214,9 @@ Box slab_box(const Geo& g, int D, bool down, int c0, const float o[3], float sAm
     bx.n[B] = nB;
     const int px = bx.n[0] | 1, pxy = (px * bx.n[1]) | 1;
     bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && pxy * bx.n[2] <= kSlabCap && bx.n[0] * bx.n[1] <= 256 * kStagePasses;
+    const float bD = g.bnd[D];
+    bx.inner = bx.staged && a0 - eps >= 0.0f && a1 + eps <= g.bnd[A] && b0 - eps >= 0.0f && b1 + eps <= g.bnd[B] &&
+               (down ? (c0 - K >= 0 && (float)(c0 + 1) <= bD) : (c0 >= 1 && (float)(c0 + K) <= bD));
     return bx;
 }
 
@@ -233,7 +241,7 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
         const float ulp = std::nextafter(cmax, INFINITY) - cmax;
         g.box_eps = 1.0f / 32.0f + 0.5f * ulp * (float)g.max_samples;
     }
-    long long checked = 0, bad = 0, waves = 0, samples = 0;
+    long long checked = 0, bad = 0, waves = 0, samples = 0, inner_samples = 0, inner_bad = 0;
     for (int vi = 0; vi < nviews; ++vi) {
         float q[4];
         stt::quat_about_y(angles[vi], q);
@@ -301,7 +309,22 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
                         float cc[3];
                         for (int c = 0; c < 3; ++c) cc[c] = med3(ln.G.pend[c], 0.0f, g.bnd[c]);
                         const int p = (int)cc[D];
-                        if (!(down ? p > bx.cend : p < bx.cend)) break;
+                        const bool member = down ? p > bx.cend : p < bx.cend;
+                        if (unchecked && bx.inner) {
+                            // inner slab: the unclamped walk takes the same samples and stops at the same one
+                            const float lim = down ? (float)(bx.cend + 1) : (float)bx.cend;
+                            const bool member_raw = down ? ln.G.pend[D] >= lim : ln.G.pend[D] < lim;
+                            if (member_raw != member) ++inner_bad;
+                            if (member) {
+                                ++inner_samples;
+                                for (int c = 0; c < 3; ++c)
+                                    if (cc[c] != ln.G.pend[c]) {
+                                        ++inner_bad;
+                                        break;
+                                    }
+                            }
+                        }
+                        if (!member) break;
                         if (unchecked) {
                             ++checked;
                             for (int c = 0; c < 3; ++c) {
@@ -330,5 +353,7 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
     out[1] = bad;
     out[2] = waves;
     out[3] = samples;
+    out[4] = inner_samples;
+    out[5] = inner_bad;
     return 0;
 }

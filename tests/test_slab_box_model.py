@@ -9,7 +9,9 @@ the floor) leaves no sample outside at the geometry where the first bound
 (0) failed on the device -- 40^3 on a 384^2 detector of 0.25 mm pixels,
 where blocks of grazing rays had all their samples in a slab beyond a face --
 nor at the other geometries below; the first bound's misses there are
-reproduced, so the model sees the failure it guards against."""
+reproduced, so the model sees the failure it guards against.  Slabs marked
+inner (walked without the clamp) must take exactly the clamped walk's
+samples at unclamped positions."""
 import ctypes as C
 import os
 import subprocess
@@ -36,9 +38,9 @@ def model(tmp_path_factory):
     def run(vol, det, spacing, ds, sid, sdd, angles, bound, oxyz=(0, 0, 0)):
         g = stt.Geometry.make(vol, det, spacing, ds, sid, sdd, oxyz)
         a = np.ascontiguousarray(angles, np.float32)
-        out = (C.c_longlong * 4)()
+        out = (C.c_longlong * 6)()
         assert lib.slab_model(C.byref(g), a.ctypes.data_as(C.POINTER(C.c_float)), len(a), bound, out) == 0
-        return dict(checked=out[0], outside=out[1], waves=out[2], samples=out[3])
+        return dict(checked=out[0], outside=out[1], waves=out[2], samples=out[3], inner=out[4], inner_bad=out[5])
 
     return run
 
@@ -52,6 +54,7 @@ def test_fine_detector_grazing_rays(model):
     assert new["outside"] == 0
     assert new["checked"] > 0.99 * new["samples"]  # nearly every sample takes the unchecked path
     assert new["samples"] == old["samples"]
+    assert new["inner_bad"] == 0
 
 
 @pytest.mark.parametrize("case", [
@@ -64,4 +67,15 @@ def test_other_geometries(model, case):
     ang = list(stt.synth_angles(1, 10, offset=3)) + [0.0, 45.0, 90.0, 135.0, 180.0, 270.0]
     new = model(vol, det, sp, ds, sid, sdd, ang, 1, oxyz)
     print(new)
-    assert new["outside"] == 0 and new["checked"] > 0
+    assert new["outside"] == 0 and new["checked"] > 0 and new["inner_bad"] == 0 and new["inner"] > 0
+
+
+def test_iteration_geometry_inner_slabs(model):
+    """256^3 on the 512x512 detector at 600/1000 mm (the iteration
+    benchmark's geometry), views on both sides of the dominant-axis switch:
+    no sample outside its box, inner slabs exact, and most samples taken on
+    inner slabs (the clamp-free walk is the common case)."""
+    new = model((256, 256, 256), (512, 512), (1, 1, 1), 1.0, 600.0, 1000.0, [17.3, 44.99, 45.01, 226.1], 1)
+    print(new, "inner fraction", new["inner"] / new["samples"])
+    assert new["outside"] == 0 and new["inner_bad"] == 0
+    assert new["inner"] > 0.5 * new["samples"]
