@@ -938,6 +938,11 @@ __device__ __forceinline__ void slab_walk(RayGen& G, const VolGeom& g, const Sla
         T.fx = __builtin_amdgcn_fractf(cx);
         T.fy = __builtin_amdgcn_fractf(cy);
         T.fz = __builtin_amdgcn_fractf(cz);
+        // The fractions are taken here, before the position advances: left
+        // to the compiler they sink below the update and each iteration ends
+        // with three register copies of the position (an empty asm statement,
+        // no instruction).
+        asm volatile("" : "+v"(T.fx), "+v"(T.fy), "+v"(T.fz));
         T.r00 = make_float2(b0[0], b0[1]);
         T.r10 = make_float2(b0[bx.px], b0[bx.px + 1]);
         T.r01 = make_float2(b0[bx.pxy], b0[bx.pxy + 1]);
