@@ -800,6 +800,39 @@ struct stt_ctx {
         return build_tables();
     }
 
+    // The x <-> z transposed volumes feed only the level-synchronous
+    // projector's x-dominant views (k_transpose_xz, then k_fp_level reading
+    // volT), so they exist only while a level shape is selected: (F+1) * V
+    // floats, 9.1 GB at 512^3 x 16 frames, that the slab projector never reads.
+    int ensure_transposed() {
+        if (xT[0]) return STT_OK;
+        for (int f = 0; f <= F; ++f) {
+            const int rc = alloc(&xT[f], VpadT);
+            if (rc) {
+                (void)drop_transposed();
+                return rc;
+            }
+        }
+        tables_dirty = true;
+        invalidate_graph();
+        return STT_OK;
+    }
+    int drop_transposed() {
+        bool any = false;
+        for (float* a : xT) any = any || a;
+        if (!any) return STT_OK;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (float*& a : xT) {
+            if (!a) continue;
+            allocs.erase(std::find(allocs.begin(), allocs.end(), (void*)a));
+            HIPCHK(hipFree(a));
+            a = nullptr;
+        }
+        tables_dirty = true;
+        invalidate_graph();
+        return STT_OK;
+    }
+
     // Release the stored trace end points (stt_set_trace_storage(ctx, 0)).
     int drop_traces() {
         if (trace_bw.empty()) return STT_OK;
@@ -971,7 +1004,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     c->derive_consts();
     const int F = nframes;
     c->x.resize(F + 1);
-    c->xT.resize(F + 1);
+    c->xT.assign(F + 1, nullptr);  // allocated while a level projector shape is selected
     c->xbar.resize(F + 1);
     c->xprev.resize(F + 1);
     c->ytv.resize(F + 1);
@@ -983,7 +1016,6 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
     c->corr_f.resize(F);
     for (int f = 0; f <= F; ++f) {
         if ((rc = c->alloc(&c->x[f], c->Vpad))) return bail(rc);
-        if ((rc = c->alloc(&c->xT[f], c->VpadT))) return bail(rc);
         if ((rc = c->alloc(&c->xbar[f], c->V))) return bail(rc);
         if ((rc = c->alloc(&c->xprev[f], c->V))) return bail(rc);
         if ((rc = c->alloc(&c->ytv[f], 4 * c->V))) return bail(rc);  // 3 spatial + temporal (STV) planes
@@ -1728,6 +1760,8 @@ int stt_set_projector_tile(stt_ctx* ctx, int rays_per_row) {
     CTX_CHECK(ctx);
     if (rays_per_row != 0 && rays_per_row != 64 && rays_per_row != 32 && rays_per_row != 16 && rays_per_row != 8)
         return fail(STT_ERR_INVALID, "projector shape must be 0 (slab), 64, 32, 16 or 8");
+    const int rc = rays_per_row ? ctx->ensure_transposed() : ctx->drop_transposed();
+    if (rc) return rc;
     ctx->vg.fp_tile = rays_per_row;
     ctx->invalidate_graph();
     return STT_OK;
@@ -1739,6 +1773,7 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     if (rc) return rc;
     const int shapes[5] = {0, 64, 32, 16, 8};
     const size_t D = ctx->D;
+    if ((rc = ctx->ensure_transposed())) return rc;  // the level shapes under test need the copies
     // The launches a sweep issues: for view index k, the layout copy and the
     // SART projector over every frame's view k (batched form), here for the
     // first few k.  Corrections and ray lengths go to scratch buffers, so no
@@ -1801,6 +1836,7 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     ctx->vg.fp_tile = best;
     if (best != old) ctx->invalidate_graph();
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (best == 0 && (rc = ctx->drop_transposed())) return rc;
     if (rays_per_row) *rays_per_row = best;
     return STT_OK;
 }
