@@ -58,7 +58,7 @@ extern "C" {
 /* Scan and volume geometry: the geometric part of the ST_Tomography
  * constructor parameter set (ST_Tomography.h:27-29, main.cpp:26-37). */
 typedef struct stt_geometry {
-    int    vol[3];      /* volume size in voxels (main.cpp -f, "volxyz") */
+    int    vol[3];      /* volume size in voxels (main.cpp -f, "volxyz"), each 4..16384 */
     float  spacing[3];  /* voxel spacing in mm (main.cpp -c; VolumeData::m_Spacings) */
     int    det[2];      /* detector W, H in pixels (main.cpp -u, "imgwh") */
     double ds;          /* detector pitch in mm (main.cpp -d) */

@@ -893,6 +893,7 @@ int valid_geometry(const stt_geometry* g) {
     if (!g) return fail(STT_ERR_INVALID, "null geometry");
     for (int c = 0; c < 3; ++c) {
         if (g->vol[c] < 4) return fail(STT_ERR_INVALID, "volume dims must be >= 4");
+        if (g->vol[c] > 16384) return fail(STT_ERR_INVALID, "volume dims must be <= 16384");
         if (!(g->spacing[c] > 0.0f)) return fail(STT_ERR_INVALID, "voxel spacing must be > 0");
     }
     if (g->det[0] < 5 || g->det[1] < 5) return fail(STT_ERR_INVALID, "detector must be at least 5x5");

@@ -721,32 +721,48 @@ struct SlabBox {
 };
 
 // A slab box passes from the wave that computes it to the whole block
-// through an LDS slot of kBoxInts ints, read back as wave-uniform values.
-constexpr int kBoxInts = 14;
+// through an LDS slot of kBoxInts packed ints, read back as wave-uniform
+// (scalar) values: 16-bit lo[0..2] and n[0..2] (every volume dimension is
+// at most 16384, so a staged box's values fit), then cbeg as a signed
+// 24-bit value (|cbeg| <= nx + ny + nz + kSlabK), the depth K = |cend - cbeg|
+// (at most kSlabK) and the two flags.  The row and plane pitches and the
+// base index follow from them on the scalar unit.  Boxes that are not staged
+// use only cbeg and cend.
+constexpr int kBoxInts = 4;
 __device__ __forceinline__ void box_put(int* slot, const SlabBox& b) {
     if ((threadIdx.x & 63) == 0) {
-        const int v[kBoxInts] = {b.lo[0], b.lo[1], b.lo[2], b.n[0], b.n[1], b.n[2], b.cbeg,
-                                 b.cend,  b.px,    b.pxy,   b.base, (int)b.staged, (int)b.inner, 0};
+        const int K = abs(b.cend - b.cbeg);
+        const unsigned w[kBoxInts] = {
+            ((unsigned)b.lo[0] & 0xFFFFu) | ((unsigned)b.lo[1] << 16),
+            ((unsigned)b.lo[2] & 0xFFFFu) | ((unsigned)b.n[2] << 16),
+            ((unsigned)b.n[0] & 0xFFFFu) | ((unsigned)b.n[1] << 16),
+            ((unsigned)b.cbeg & 0xFFFFFFu) | ((unsigned)(K & 0x1F) << 24) | ((unsigned)b.staged << 29) |
+                ((unsigned)b.inner << 30)};
 #pragma unroll
-        for (int k = 0; k < kBoxInts; ++k) slot[k] = v[k];
+        for (int k = 0; k < kBoxInts; ++k) slot[k] = (int)w[k];
     }
 }
+template <bool DOWN>
 __device__ __forceinline__ SlabBox box_get(const int* slot) {
-    int v[kBoxInts];
-#pragma unroll
-    for (int k = 0; k < kBoxInts; ++k) v[k] = __builtin_amdgcn_readfirstlane(slot[k]);
+    const unsigned v0 = (unsigned)__builtin_amdgcn_readfirstlane(slot[0]);
+    const unsigned v1 = (unsigned)__builtin_amdgcn_readfirstlane(slot[1]);
+    const unsigned v2 = (unsigned)__builtin_amdgcn_readfirstlane(slot[2]);
+    const unsigned v3 = (unsigned)__builtin_amdgcn_readfirstlane(slot[3]);
     SlabBox b;
-    for (int c = 0; c < 3; ++c) {
-        b.lo[c] = v[c];
-        b.n[c] = v[3 + c];
-    }
-    b.cbeg = v[6];
-    b.cend = v[7];
-    b.px = v[8];
-    b.pxy = v[9];
-    b.base = v[10];
-    b.staged = v[11] != 0;
-    b.inner = v[12] != 0;
+    b.lo[0] = (int)(v0 & 0xFFFFu);
+    b.lo[1] = (int)(v0 >> 16);
+    b.lo[2] = (int)(v1 & 0xFFFFu);
+    b.n[0] = (int)(v2 & 0xFFFFu);
+    b.n[1] = (int)(v2 >> 16);
+    b.n[2] = (int)(v1 >> 16);
+    b.cbeg = (int)(v3 << 8) >> 8;
+    const int K = (int)((v3 >> 24) & 0x1Fu);
+    b.cend = DOWN ? b.cbeg - K : b.cbeg + K;
+    b.staged = ((v3 >> 29) & 1u) != 0;
+    b.inner = ((v3 >> 30) & 1u) != 0;
+    b.px = b.n[0] | 1;
+    b.pxy = (b.px * b.n[1]) | 1;
+    b.base = b.lo[0] + b.px * b.lo[1] + b.pxy * b.lo[2];
     return b;
 }
 
@@ -1055,7 +1071,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     }
 #pragma unroll 1
     for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
-        const SlabBox bx = box_get(slots + kBoxInts * (guard & 1));
+        const SlabBox bx = box_get<DOWN>(slots + kBoxInts * (guard & 1));
         if (bx.staged) slab_stage<NT>(vv, g, bx, box);
         if (wv == (guard & (NT / 64 - 1))) {
             SlabBox nb = slab_box<D, DOWN>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);

@@ -44,6 +44,9 @@ def test_invalid_arguments_are_rejected_before_device_use():
     g = stt.Geometry.make((2, 32, 32), (64, 64))  # dims < 4
     assert stt.lib.stt_ctx_create(C.byref(g), C.byref(p), 1, P, C.byref(out)) == stt.STT_ERR_INVALID
     assert b"dims" in stt.lib.stt_last_error()
+    g = stt.Geometry.make((16385, 8, 8), (64, 64))  # the slab projector packs box extents in 16 bits
+    assert stt.lib.stt_ctx_create(C.byref(g), C.byref(p), 1, P, C.byref(out)) == stt.STT_ERR_INVALID
+    assert b"16384" in stt.lib.stt_last_error()
     g = stt.Geometry.make((32, 32, 32), (64, 64), spacing=(1, 0, 1))
     assert stt.lib.stt_ctx_create(C.byref(g), C.byref(p), 1, P, C.byref(out)) == stt.STT_ERR_INVALID
     g = stt.Geometry.make((32, 32, 32), (64, 64))
