@@ -20,7 +20,9 @@
  *   - Host pointers are copied in or out; "_dev" variants take device pointers
  *     that the library borrows and never frees.
  *   - One context per thread.  All device work of a context is issued on the
- *     context's own HIP stream; functions that return data synchronise it.
+ *     context's own HIP streams (its stream, plus seven frame-group streams
+ *     that the batched iteration forks onto and joins before it returns);
+ *     functions that return data synchronise them.
  */
 #ifndef STT_H
 #define STT_H
