@@ -500,8 +500,8 @@ def main():
                     help="SART views in the CPU-baseline sample (default: all views of one frame)")
     ap.add_argument("--cpu-lean", action="store_true", help="also time the lean CPU variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--projector-tile", type=int, default=-1,
-                    help="projector shape (0 slab, 64/32/16/8 level-synchronous); default: autotune")
+    ap.add_argument("--projector-tile", type=int, default=None,
+                    help="projector shape (0, -32, -8 slab; 64/32/16/8 level-synchronous); default: autotune")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (counter calibration)")
     ap.add_argument("--flow", action="store_true",
@@ -540,13 +540,13 @@ def main():
     t0 = time.time()
     g, p, ctx, angles, projections = setup(stt, cfg)
     log(f"setup done in {time.time() - t0:.1f}s")
-    if args.projector_tile >= 0:
+    if args.projector_tile is not None:
         tile = args.projector_tile
         ctx.set_projector_tile(tile)
         log(f"projector tile {tile} (fixed)")
     else:
         tile, tile_ms = ctx.autotune()
-        log(f"autotune: projector tile {tile} (ms per batched SART launch, shapes 0/64/32/16/8: "
+        log(f"autotune: projector tile {tile} (ms per batched SART launch, shapes 0/-32/-8/64/32/16/8: "
             f"{[round(v, 4) for v in tile_ms]})")
     batched = args.form != "plain"
     use_graph = args.form == "batched-graph"

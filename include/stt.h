@@ -263,15 +263,19 @@ int  stt_group_destroy(stt_group* grp);
 int  stt_group_iterate(stt_group* grp, int n_iters, int use_graph, double* total_ms);
 /* Projector shape of this context (other contexts keep theirs; the context's
  * captured graph is rebuilt): 0 = slab projector (16x16 rays per block,
- * volume boxes staged in LDS; default); 64, 32, 16 or 8 = level-synchronous
- * projector with that many rays per detector row of a 64-lane wave. */
+ * volume boxes staged in LDS; default); -32 or -8 = slab projector with 32x8
+ * or 8x32 rays per block; 64, 32, 16 or 8 = level-synchronous projector with
+ * that many rays per detector row of a 64-lane wave (these keep an x<->z
+ * transposed copy of every frame volume while selected).  Sample counts and
+ * ray lengths do not depend on the shape. */
+#define STT_PROJECTOR_SHAPES 7
 int  stt_set_projector_tile(stt_ctx* ctx, int rays_per_row);
 /* Times every projector shape on the launches a SART sweep issues (for the
  * first view indices k: the layout copy and the projector over every frame's
  * view k, corrections and ray lengths into scratch buffers, so no solver
  * state changes) and keeps the fastest; *rays_per_row (may be NULL) receives
- * the choice and ms_per_shape[5] (may be NULL) the time per view index of
- * shapes 0/64/32/16/8. */
+ * the choice and ms_per_shape[STT_PROJECTOR_SHAPES] (may be NULL) the time
+ * per view index of shapes 0/-32/-8/64/32/16/8. */
 int  stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape);
 
 /* Measurement helper: the projector launches of one iteration (every

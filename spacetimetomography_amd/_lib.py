@@ -23,6 +23,8 @@ STT_BP_VOXEL, STT_BP_RAY = 0, 1
 STT_PRIOR_ATV, STT_PRIOR_TV, STT_PRIOR_STV = 0, 1, 2
 STT_INTERP_CUBIC_CLAMPED, STT_INTERP_TRILINEAR = 0, 1
 STT_BUF_XBAR, STT_BUF_XPREV, STT_BUF_YTV, STT_BUF_YTP, STT_BUF_YBC, STT_BUF_FLOW = range(6)
+PROJECTOR_SHAPES = (0, -32, -8, 64, 32, 16, 8)  # stt_autotune order (STT_PROJECTOR_SHAPES)
+PROJECTOR_SHAPES_N = len(PROJECTOR_SHAPES)
 
 
 class Geometry(C.Structure):
@@ -522,7 +524,7 @@ class Context:
 
     def autotune(self):
         best = C.c_int()
-        ms = (C.c_double * 5)()
+        ms = (C.c_double * PROJECTOR_SHAPES_N)()
         check(lib.stt_autotune(self.h, C.byref(best), ms))
         return best.value, list(ms)
 

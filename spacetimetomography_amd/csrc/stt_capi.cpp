@@ -1759,9 +1759,10 @@ int stt_group_iterate(stt_group* grp, int n_iters, int use_graph, double* total_
 
 int stt_set_projector_tile(stt_ctx* ctx, int rays_per_row) {
     CTX_CHECK(ctx);
-    if (rays_per_row != 0 && rays_per_row != 64 && rays_per_row != 32 && rays_per_row != 16 && rays_per_row != 8)
-        return fail(STT_ERR_INVALID, "projector shape must be 0 (slab), 64, 32, 16 or 8");
-    const int rc = rays_per_row ? ctx->ensure_transposed() : ctx->drop_transposed();
+    if (rays_per_row != 0 && rays_per_row != -32 && rays_per_row != -8 && rays_per_row != 64 && rays_per_row != 32 &&
+        rays_per_row != 16 && rays_per_row != 8)
+        return fail(STT_ERR_INVALID, "projector shape must be 0, -32 or -8 (slab), 64, 32, 16 or 8 (level-synchronous)");
+    const int rc = rays_per_row > 0 ? ctx->ensure_transposed() : ctx->drop_transposed();
     if (rc) return rc;
     ctx->vg.fp_tile = rays_per_row;
     ctx->invalidate_graph();
@@ -1772,7 +1773,7 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     CTX_CHECK(ctx);
     int rc = ctx->ready_check();
     if (rc) return rc;
-    const int shapes[5] = {0, 64, 32, 16, 8};
+    const int shapes[STT_PROJECTOR_SHAPES] = {0, -32, -8, 64, 32, 16, 8};
     const size_t D = ctx->D;
     if ((rc = ctx->ensure_transposed())) return rc;  // the level shapes under test need the copies
     // The launches a sweep issues: for view index k, the layout copy and the
@@ -1816,7 +1817,7 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
         }
         return hipSuccess;
     };
-    for (int si = 0; si < 5; ++si) {
+    for (int si = 0; si < STT_PROJECTOR_SHAPES; ++si) {
         ctx->vg.fp_tile = shapes[si];
         HIPCHK(sweep());  // warm-up
         HIPCHK(hipEventRecord(e0, ctx->stream));
@@ -1837,7 +1838,7 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
     ctx->vg.fp_tile = best;
     if (best != old) ctx->invalidate_graph();
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (best == 0 && (rc = ctx->drop_transposed())) return rc;
+    if (best <= 0 && (rc = ctx->drop_transposed())) return rc;
     if (rays_per_row) *rays_per_row = best;
     return STT_OK;
 }

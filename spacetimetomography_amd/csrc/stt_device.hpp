@@ -42,7 +42,8 @@ struct VolGeom {
     float box_eps;       // bound (cells) on how far a sample drifts off its exact line: max_samples
                          // rounded additions at coordinates of magnitude <= 2*max(dims), plus slack
     int acc_wshift;      // ray-BP weight sums are int64 multiples of 2^-acc_wshift (see k_raybp_prep)
-    int fp_tile;         // projector shape of this context (host-side launch choice): 0 = slab projector,
+    int fp_tile;         // projector shape of this context (host-side launch choice): 0 = slab projector
+                         // (16x16 rays per block), -32 / -8 = slab projector with 32x8 / 8x32 rays per block,
                          // 64/32/16/8 = level-synchronous projector with that many rays per wave row
     unsigned vol_bytes;  // bytes of one padded volume allocation (buffer range)
     unsigned volT_bytes; // bytes of one padded transposed (z-fastest) volume

@@ -2465,15 +2465,22 @@ static inline dim3 vol_grid(const VolGeom& g, int nitems) {
 
 // Projector shape, VolGeom::fp_tile of the launching context (set by
 // stt_set_projector_tile() / the tuner): 0 = slab projector (16x16 rays per
-// block, LDS-staged volume boxes); 64, 32, 16, 8 = level-synchronous
+// block, LDS-staged volume boxes), -32 / -8 = slab projector with 32x8 / 8x32
+// rays per block; 64, 32, 16, 8 = level-synchronous
 // projector with that many rays per wave row (64x4, 32x8, 16x16 or 8x32 rays
 // per 256-thread block, transposed copy for x-dominant views in the SART
 // sweep).
 
 template <int MODE>
 static hipError_t launch_fp_slab(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
-    const dim3 grid(((g.W + 15) / 16) * ((g.H + 15) / 16), nitems);
-    hipLaunchKernelGGL((k_fp_slab<MODE, 16, 16>), grid, dim3(256), 0, s, items, g, factor);
+    // fp_tile 0: 16 x 16 rays per block; -32: 32 x 8; -8: 8 x 32
+    const int tw = g.fp_tile == 0 ? 16 : -g.fp_tile, th = 256 / tw;
+    const dim3 grid(((g.W + tw - 1) / tw) * ((g.H + th - 1) / th), nitems);
+    switch (tw) {
+        case 32: hipLaunchKernelGGL((k_fp_slab<MODE, 32, 8>), grid, dim3(256), 0, s, items, g, factor); break;
+        case 8: hipLaunchKernelGGL((k_fp_slab<MODE, 8, 32>), grid, dim3(256), 0, s, items, g, factor); break;
+        default: hipLaunchKernelGGL((k_fp_slab<MODE, 16, 16>), grid, dim3(256), 0, s, items, g, factor); break;
+    }
     return hipGetLastError();
 }
 
@@ -2496,10 +2503,10 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
     dim3 grid(nb, nitems);
     switch (mode) {
         case FP_MODE_FP:
-            return g.fp_tile == 0 ? launch_fp_slab<FP_MODE_FP>(items, nitems, g, factor, s)
+            return g.fp_tile <= 0 ? launch_fp_slab<FP_MODE_FP>(items, nitems, g, factor, s)
                                   : launch_fp_level<FP_MODE_FP>(items, nitems, g, factor, s);
         case FP_MODE_SART:
-            return g.fp_tile == 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
+            return g.fp_tile <= 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
                                   : launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
         case FP_MODE_RAYBP: hipLaunchKernelGGL((k_fp<FP_MODE_RAYBP, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
@@ -2509,7 +2516,7 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom& g, hipStream_t s) {
-    if (g.fp_tile == 0) return hipSuccess;
+    if (g.fp_tile <= 0) return hipSuccess;  // slab projectors read the x-fastest volume
     const dim3 grid((g.nx + 31) / 32, (g.nz + 31) / 32, g.ny * nitems);
     hipLaunchKernelGGL(k_transpose_xz, grid, dim3(256), 0, s, items, g);
     return hipGetLastError();

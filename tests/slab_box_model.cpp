@@ -228,7 +228,11 @@ struct Lane {
 
 }  // namespace
 
-extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nviews, int bound, long long* out) {
+// tw: rays per tile row (16, 32 or 8; 256 / tw rows), the k_fp_slab tile.
+extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nviews, int bound, int tw,
+                          long long* out) {
+    if (tw != 16 && tw != 32 && tw != 8) return 1;
+    const int th = 256 / tw;
     Geo g;
     for (int c = 0; c < 3; ++c) g.dims[c] = geom->vol[c];
     g.W = geom->det[0];
@@ -250,7 +254,7 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
         const int D = march & 3;
         const bool down = (march & 4) != 0;
         const int A = D == 0 ? 1 : 0, B = D == 2 ? 1 : 2;
-        const int tiles_x = (g.W + 15) / 16, tiles_y = (g.H + 15) / 16;
+        const int tiles_x = (g.W + tw - 1) / tw, tiles_y = (g.H + th - 1) / th;
         for (int tile = 0; tile < tiles_x * tiles_y; ++tile) {
             const int tx = tile % tiles_x, ty = tile / tiles_x;
             std::vector<Lane> L(256);
@@ -258,7 +262,7 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
                            down ? -INFINITY : INFINITY};
             float o[3] = {v.o[0], v.o[1], v.o[2]};
             for (int t = 0; t < 256; ++t) {
-                const int i = tx * 16 + t % 16, j = ty * 16 + t / 16;
+                const int i = tx * tw + t % tw, j = ty * th + t / tw;
                 const bool valid = i < g.W && j < g.H;
                 Ray r;
                 produce_ray(v, g, (float)(valid ? i : 0), (float)(valid ? j : 0), r);

@@ -31,7 +31,7 @@ def _ctx(vol, det, spacing=(1, 1, 1), ds=1.0, sid=60.0, sdd=100.0, oxyz=(0, 0, 0
     return g, stt.Context(g, p, list(P))
 
 
-PROJECTOR_SHAPES = [0, 64, 32, 16, 8]  # slab projector, level-synchronous tiles
+PROJECTOR_SHAPES = [0, -32, -8, 64, 32, 16, 8]  # slab projector (three tiles), level-synchronous tiles
 
 
 @pytest.mark.parametrize("case", range(len(FP_CASES)))
@@ -67,7 +67,7 @@ def test_fp_shapes_agree_at_full_size():
     straight from HBM.  Views around the 45-degree switch of the dominant axis
     included.  Sample counts must be identical and values agree to fp32
     rounding; two views are also checked against the oracle."""
-    print("[fp-full] 256^3, 512x512, slab vs level-16 vs oracle", flush=True)
+    print("[fp-full] 256^3, 512x512, slab (16x16, 32x8, 8x32) vs level-16 vs oracle", flush=True)
     dims, det = (256, 256, 256), (512, 512)
     angles = [0.0, 17.3, 44.99, 45.0, 45.01, 90.0, 134.9, 180.0, 226.1, 271.7, 315.0, 359.9]
     g = stt.Geometry.make(dims, det, (1, 1, 1), 1.0, 600.0, 1000.0)
@@ -77,14 +77,15 @@ def test_fp_shapes_agree_at_full_size():
     ctx.set_views(0, angles)
     try:
         for v, a in enumerate(angles):
-            ctx.set_projector_tile(0)
-            ax0, rl0, w0 = ctx.forward_project(0, v)
             ctx.set_projector_tile(16)
             ax1, rl1, w1 = ctx.forward_project(0, v)
-            assert np.array_equal(w0, w1), (a, np.count_nonzero(w0 != w1))
-            assert np.array_equal(rl0, rl1), a
-            e = rel_l2(ax0, ax1)
-            assert e <= 1e-6, (a, e)
+            for slab in (-32, -8, 0):
+                ctx.set_projector_tile(slab)
+                ax0, rl0, w0 = ctx.forward_project(0, v)
+                assert np.array_equal(w0, w1), (a, slab, np.count_nonzero(w0 != w1))
+                assert np.array_equal(rl0, rl1), (a, slab)
+                e = rel_l2(ax0, ax1)
+                assert e <= 1e-6, (a, slab, e)
             if v in (1, 4):
                 oax, _, ow = O.fp(g, O.quat_y(a), x)
                 assert np.array_equal(w0, ow), a
@@ -110,13 +111,14 @@ def test_slab_equals_level_on_fine_detector():
     ctx.set_volume(0, stt.synth_phantom_frame(dims, 0, n_ellipsoids=6, seed=8, deform_seed=2, amp=1.5))
     try:
         for k in range(12):
-            ctx.set_projector_tile(0)
-            a0, r0, w0 = ctx.forward_project(0, k)
             ctx.set_projector_tile(16)
             a1, r1, w1 = ctx.forward_project(0, k)
-            assert np.array_equal(w0, w1) and np.array_equal(r0, r1), k
-            err = float(np.max(np.abs(a0 - a1)) / max(float(np.max(np.abs(a1))), 1e-30))
-            assert err <= 1e-6, (k, ang[k], err)
+            for slab in (0, -32, -8):
+                ctx.set_projector_tile(slab)
+                a0, r0, w0 = ctx.forward_project(0, k)
+                assert np.array_equal(w0, w1) and np.array_equal(r0, r1), (k, slab)
+                err = float(np.max(np.abs(a0 - a1)) / max(float(np.max(np.abs(a1))), 1e-30))
+                assert err <= 1e-6, (k, slab, ang[k], err)
     finally:
         ctx.set_projector_tile(0)
         ctx.close()

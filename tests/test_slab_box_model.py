@@ -33,22 +33,23 @@ def model(tmp_path_factory):
                     os.path.join(ROOT, "spacetimetomography_amd", "csrc", "stt_geometry.cpp")], check=True)
     lib = C.CDLL(so)
     lib.slab_model.restype = C.c_int
-    lib.slab_model.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_longlong)]
+    lib.slab_model.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
 
-    def run(vol, det, spacing, ds, sid, sdd, angles, bound, oxyz=(0, 0, 0)):
+    def run(vol, det, spacing, ds, sid, sdd, angles, bound, oxyz=(0, 0, 0), tw=16):
         g = stt.Geometry.make(vol, det, spacing, ds, sid, sdd, oxyz)
         a = np.ascontiguousarray(angles, np.float32)
         out = (C.c_longlong * 6)()
-        assert lib.slab_model(C.byref(g), a.ctypes.data_as(C.POINTER(C.c_float)), len(a), bound, out) == 0
+        assert lib.slab_model(C.byref(g), a.ctypes.data_as(C.POINTER(C.c_float)), len(a), bound, tw, out) == 0
         return dict(checked=out[0], outside=out[1], waves=out[2], samples=out[3], inner=out[4], inner_bad=out[5])
 
     return run
 
 
-def test_fine_detector_grazing_rays(model):
+@pytest.mark.parametrize("tw", [16, 32, 8])
+def test_fine_detector_grazing_rays(model, tw):
     ang = list(stt.synth_angles(2, 12, offset=0))
-    old = model((40, 40, 40), (384, 384), (1, 1, 1), 0.25, 600.0, 1000.0, ang, 0)
-    new = model((40, 40, 40), (384, 384), (1, 1, 1), 0.25, 600.0, 1000.0, ang, 1)
+    old = model((40, 40, 40), (384, 384), (1, 1, 1), 0.25, 600.0, 1000.0, ang, 0, tw=tw)
+    new = model((40, 40, 40), (384, 384), (1, 1, 1), 0.25, 600.0, 1000.0, ang, 1, tw=tw)
     print(old, new)
     assert old["outside"] > 0           # the failure seen on the device
     assert new["outside"] == 0
@@ -57,25 +58,27 @@ def test_fine_detector_grazing_rays(model):
     assert new["inner_bad"] == 0
 
 
+@pytest.mark.parametrize("tw", [16, 32, 8])
 @pytest.mark.parametrize("case", [
     ((64, 48, 56), (200, 160), (1.0, 1.0, 1.0), 0.5, 120.0, 200.0, (0.0, 0.0, 0.0)),
     ((40, 36, 44), (128, 112), (0.9, 1.2, 1.0), 0.6, 80.0, 140.0, (2.0, -1.5, 1.0)),
     ((32, 32, 32), (96, 96), (1.0, 1.0, 1.0), 0.2, 60.0, 100.0, (0.0, 3.0, 0.0)),
 ])
-def test_other_geometries(model, case):
+def test_other_geometries(model, case, tw):
     vol, det, sp, ds, sid, sdd, oxyz = case
     ang = list(stt.synth_angles(1, 10, offset=3)) + [0.0, 45.0, 90.0, 135.0, 180.0, 270.0]
-    new = model(vol, det, sp, ds, sid, sdd, ang, 1, oxyz)
+    new = model(vol, det, sp, ds, sid, sdd, ang, 1, oxyz, tw=tw)
     print(new)
     assert new["outside"] == 0 and new["checked"] > 0 and new["inner_bad"] == 0 and new["inner"] > 0
 
 
-def test_iteration_geometry_inner_slabs(model):
+@pytest.mark.parametrize("tw", [16, 32, 8])
+def test_iteration_geometry_inner_slabs(model, tw):
     """256^3 on the 512x512 detector at 600/1000 mm (the iteration
     benchmark's geometry), views on both sides of the dominant-axis switch:
     no sample outside its box, inner slabs exact, and most samples taken on
     inner slabs (the clamp-free walk is the common case)."""
-    new = model((256, 256, 256), (512, 512), (1, 1, 1), 1.0, 600.0, 1000.0, [17.3, 44.99, 45.01, 226.1], 1)
+    new = model((256, 256, 256), (512, 512), (1, 1, 1), 1.0, 600.0, 1000.0, [17.3, 44.99, 45.01, 226.1], 1, tw=tw)
     print(new, "inner fraction", new["inner"] / new["samples"])
     assert new["outside"] == 0 and new["inner_bad"] == 0
     assert new["inner"] > 0.5 * new["samples"]
