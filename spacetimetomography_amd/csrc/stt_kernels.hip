@@ -1256,13 +1256,15 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 // centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
 // clamp [0, G_Max].  Atomic-free, one thread per voxel column segment.
 //
-// A 256-thread block owns a 64 x 4 x 16 voxel tile (lane = x, wave = y, each
+// A 256-thread block owns a 32 x 8 x 16 voxel tile (a wave's two half-waves
+// take rows y0 + 2w and y0 + 2w + 1, lane = x within a half-wave, each
 // thread 16 consecutive z).  The tile is a box, so the detector positions of
 // its voxels lie inside the bounding box of its eight corners' projections;
 // the block stages that box of the correction image in LDS with row loads and
 // reads every voxel's 4x4 window from there: 16 taps per voxel from LDS
 // instead of four unaligned 16-byte global rows.  Stacking the tile along z
-// keeps the box small (about 1.3-1.6 detector pixels staged per voxel).
+// keeps the box small: it grows with the tile's x and y extent, not with
+// its 16 z steps.
 //
 // Blocks whose box lies at least two pixels inside the [2, W-2) x [2, H-2)
 // window and fits in LDS take a fast path: every voxel is updated and its
@@ -1270,8 +1272,8 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 // corner projections).  Every other block takes the general path: per-voxel
 // skip test with the reference's own arithmetic near the window edges, and a
 // window read from global memory when it is not inside the staged box.
-constexpr int kBpTileX = 64;
-constexpr int kBpTileY = 4;
+constexpr int kBpTileX = 32;  // 64 x 4 staged 22% fewer blocks at 512^3 / 1024^2 (the rest read global memory);
+constexpr int kBpTileY = 8;   // 16 x 16 was slower at both sizes
 constexpr int kBpVoxZ = 16;  // voxels per thread along z (8 and 32 measured slower)
 constexpr int kBpLdsFloats = 4096;  // 16 KB staged detector footprint per block (22 KB: the iteration 3% slower, as
                                     // fewer blocks share a CU with the other streams' projector blocks; 12 KB: more fall back)
@@ -1347,7 +1349,7 @@ __global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ i
 
     // This thread's voxels are loaded before the footprint, so the two
     // latencies overlap.
-    const int x = x0 + lane, y = y0 + wv;
+    const int x = x0 + (lane & 31), y = y0 + 2 * wv + (lane >> 5);
     const int nz_here = (x < g.nx && y < g.ny) ? min(kBpVoxZ, g.nz - z0) : 0;
     const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)it.vol, (short)0, (int)g.vol_bytes, kBufWord3);
     const int vo = 4 * (x + g.nx * y);
