@@ -800,6 +800,12 @@ struct stt_ctx {
         return build_tables();
     }
 
+    // Takes an allocation off the list freed at destruction (the caller frees it).
+    void forget_alloc(const void* a) {
+        const auto at = std::find(allocs.begin(), allocs.end(), a);
+        if (at != allocs.end()) allocs.erase(at);
+    }
+
     // The x <-> z transposed volumes feed only the level-synchronous
     // projector's x-dominant views (k_transpose_xz, then k_fp_level reading
     // volT), so they exist only while a level shape is selected: (F+1) * V
@@ -822,14 +828,17 @@ struct stt_ctx {
         for (float* a : xT) any = any || a;
         if (!any) return STT_OK;
         HIPCHK(hipStreamSynchronize(stream));
+        hipError_t first = hipSuccess;  // every copy is released and forgotten even if one free fails
         for (float*& a : xT) {
             if (!a) continue;
-            allocs.erase(std::find(allocs.begin(), allocs.end(), (void*)a));
-            HIPCHK(hipFree(a));
+            forget_alloc(a);
+            const hipError_t e = hipFree(a);
+            if (first == hipSuccess) first = e;
             a = nullptr;
         }
         tables_dirty = true;
         invalidate_graph();
+        HIPCHK(first);
         return STT_OK;
     }
 
@@ -837,16 +846,19 @@ struct stt_ctx {
     int drop_traces() {
         if (trace_bw.empty()) return STT_OK;
         HIPCHK(hipStreamSynchronize(stream));
+        hipError_t first = hipSuccess;  // every buffer is released and forgotten even if one free fails
         for (int f = 0; f < F; ++f)
             for (float* a : {trace_bw[f], trace_fw[f]}) {
-                allocs.erase(std::find(allocs.begin(), allocs.end(), (void*)a));
-                HIPCHK(hipFree(a));
+                forget_alloc(a);
+                const hipError_t e = hipFree(a);
+                if (first == hipSuccess) first = e;
             }
         trace_bw.clear();
         trace_fw.clear();
         trace_gen.assign(F, 0);
         tables_dirty = true;
         invalidate_graph();
+        HIPCHK(first);
         return STT_OK;
     }
 
