@@ -9,6 +9,13 @@
 // R = float reproduces the reference's mixed precision (double literals in
 // cubicInterpolate, double m_halfFrustumWidth/m_ds/m_alpha); R = double is
 // the FP64 oracle (all arithmetic in double).
+//
+// Parity unpinned: the optical-flow solver's stopping residual
+// err = (d^2 + p^2) / nPx of its primal and dual residual sums
+// (optical_flow.h:359-362, 465-486, 500-502), which sets the iteration counts,
+// is restated without an independent check; the solver's fixed point, its
+// warp, gradient, blur and resize are pinned (closed forms, a duality-gap
+// certificate, and the reference's own code compiled in oracle/_ref).
 #include "stt_oracle.h"
 
 #include <algorithm>

@@ -155,3 +155,96 @@ def test_of_multiscale_level_sizes_and_translation():
     for c in range(3):
         m = float(V[c * n:(c + 1) * n].reshape(dims[::-1])[inner].mean())
         assert abs(m - s[c]) < 0.15, (c, m, s[c])
+
+
+def _fwd(a, dims, ax):
+    """tools.cpp forward difference along x/y/z (0 at the last index)."""
+    a = a.reshape(dims[::-1])
+    out = np.zeros_like(a)
+    axis = 2 - ax
+    lo, hi = [slice(None)] * 3, [slice(None)] * 3
+    lo[axis], hi[axis] = slice(0, -1), slice(1, None)
+    out[tuple(lo)] = a[tuple(hi)] - a[tuple(lo)]
+    return out.ravel()
+
+
+def _bwd_interior(a, dims, ax):
+    """a[i] - a[i-1] (the backward difference away from the first index)."""
+    a = a.reshape(dims[::-1])
+    out = np.zeros_like(a)
+    axis = 2 - ax
+    lo, hi = [slice(None)] * 3, [slice(None)] * 3
+    lo[axis], hi[axis] = slice(0, -1), slice(1, None)
+    out[tuple(hi)] = a[tuple(hi)] - a[tuple(lo)]
+    return out.ravel()
+
+
+@pytest.mark.parametrize("norm", [4, 1])
+def test_of_level_fixed_point_has_a_zero_duality_gap(norm):
+    """The solver loop (optical_flow.h:300-480) is a primal-dual iteration for
+        min_v  sum_c lam*H_eps(|D v_c|) + sum_ROI |grad(I2w) . v + ut|,
+    the Huber-TV / L1 energy its prox steps define: the TV duals are scaled
+    by 1/(1 + sigma1*eps/lam) and projected on the lam-ball per flow
+    component (:441-455; eps = 0 without Huber, norm != 4), the data dual is
+    clamped to [-1, 1] with the linearisation ut (:264, :458).  At its fixed
+    point every Fenchel-Young gap vanishes,
+        lam*H_eps(|z_c|) + eps/(2 lam)|y_c|^2 - y_c . z_c = 0,  z_c = D v_c,
+        |z4 + ut| - y4 (z4 + ut) = 0,                         z4 = grad(I2w) . v,
+    and the primal step stops moving (K^T y = 0 on the ROI).  The energy and
+    the gaps are evaluated here from the returned v and y, independently of
+    the oracle's update code: a wrong Huber factor, a lost data term or a
+    flipped divergence leaves a gap of order one, and so does a perturbed v
+    (both checked).  One warp, so the linearisation is that of the input
+    flow (zero)."""
+    dims = (14, 12, 10)
+    n = int(np.prod(dims))
+    roi = (2, 2, 2, 2, 2, 2)
+    lam, eps = 0.05, 0.1
+    I1, I2 = _smooth_pair(dims, (0.5, -0.3, 0.2))
+    V0 = np.zeros(3 * n, np.float32)
+
+    def solve(huber):
+        V, Y, _, _ = O.of_level(dims, np.asarray(dims, np.float32), I1, I2, V0, np.zeros(10 * n, np.float32), tol=0.0,
+                                lam=lam, max_iter=1000, norm=norm, warps=1, huber=huber, roi=roi)
+        return V.astype(np.float64).reshape(3, n), Y.astype(np.float64).reshape(10, n)
+
+    W = O.warp_ext(dims, np.asarray(dims, np.float32), V0, I2, -1.0)
+    box = np.zeros(dims[::-1], bool)
+    box[roi[4]:dims[2] - roi[5], roi[2]:dims[1] - roi[3], roi[0]:dims[0] - roi[1]] = True
+    inner = np.zeros(dims[::-1], bool)
+    inner[roi[4] + 1:dims[2] - roi[5] - 1, roi[2] + 1:dims[1] - roi[3] - 1, roi[0] + 1:dims[0] - roi[1] - 1] = True
+    box, inner = box.ravel(), inner.ravel()
+    u = np.stack([g.astype(np.float64) * box for g in O.gradient_central(dims, W)])
+    ut = (W.astype(np.float64) - I1) * box  # the input flow is zero
+    e_judge = eps if norm == 4 else 0.0
+
+    def certificate(v, y):
+        gap = 0.0
+        energy = 0.0
+        for c in range(3):
+            z = np.stack([_fwd(v[c], dims, ax) for ax in range(3)])
+            yc = y[3 * c:3 * c + 3]
+            s = np.sqrt((z * z).sum(0))
+            h = np.where(s <= e_judge, s * s / (2 * e_judge) if e_judge else s, s - e_judge / 2)
+            energy += lam * h[box].sum()
+            fy = lam * h + (e_judge / (2 * lam)) * (yc * yc).sum(0) - (yc * z).sum(0)
+            gap += fy[box].sum()
+        r = (u * v).sum(0) + ut
+        energy += np.abs(r[box]).sum()
+        gap += (np.abs(r) - y[9] * r)[box].sum()
+        kty = np.stack([-(_bwd_interior(y[3 * c], dims, 0) + _bwd_interior(y[3 * c + 1], dims, 1) +
+                          _bwd_interior(y[3 * c + 2], dims, 2)) + u[c] * y[9] for c in range(3)])
+        return gap / energy, float(np.abs(kty[:, inner]).max())
+
+    v, y = solve(eps)
+    assert np.abs(y[9][box]).max() <= 1.0 + 1e-6
+    for c in range(3):  # the TV duals lie in the lam-ball
+        assert np.sqrt((y[3 * c:3 * c + 3][:, box] ** 2).sum(0)).max() <= lam * (1 + 1e-5)
+    rel_gap, feas = certificate(v, y)
+    print(f"norm {norm}: relative duality gap {rel_gap:.2e}, |K^T y| {feas:.2e}")
+    assert abs(rel_gap) <= 1e-5 and feas <= 1e-5
+    # sensitivity: a perturbed flow, and (with Huber) the fixed point of another epsilon
+    rng = np.random.default_rng(0)
+    assert certificate(v + 0.02 * rng.standard_normal(v.shape) * box, y)[0] > 1e-2
+    if norm == 4:
+        assert certificate(*solve(0.5))[0] > 1e-2
