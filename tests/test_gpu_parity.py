@@ -158,6 +158,58 @@ def test_bp_voxel_matches_oracle():
     ctx.close()
 
 
+def _quat_axis(axis, deg):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    h = np.radians(deg) / 2.0
+    return np.array([*(a * np.sin(h)), np.cos(h)], np.float32)
+
+
+# (vol, det, sid, sdd): magnification about 1.5-2 (y-column boxes fit the
+# fixed LDS pitch), about 3 (runtime pitch), about 4-5 (boxes too tall for LDS:
+# the general path's global window reads)
+BP_TILE_CASES = [
+    ((64, 64, 64), (136, 136), 300.0, 500.0),
+    ((72, 48, 80), (220, 150), 180.0, 500.0),
+    ((64, 64, 64), (330, 330), 150.0, 500.0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(BP_TILE_CASES)))
+def test_bp_voxel_tiles_match_oracle(case):
+    """Both backprojector orientations against the oracle on volumes with
+    interior tiles: views about the y axis take the y-column form, tilted
+    views (a quaternion about an oblique axis) the z-column form."""
+    vol, det, sid, sdd = BP_TILE_CASES[case]
+    print(f"[bp-tiles] case={case} vol={vol} det={det} sid={sid} sdd={sdd}", flush=True)
+    rng = np.random.default_rng(70 + case)
+    qs = [O.quat_y(17.0), O.quat_y(135.0), O.quat_y(250.0),
+          _quat_axis((0.3, 1.0, 0.2), 40.0), _quat_axis((1.0, 0.4, -0.5), 200.0)]
+    g, ctx = _ctx(vol, det, sid=sid, sdd=sdd, P=(len(qs),))
+    x0 = (0.5 * rng.random(int(np.prod(vol)))).astype(np.float32)
+    ctx.set_view_quaternions(0, np.stack(qs))
+    gmax = ctx.gmax()
+    for v, q in enumerate(qs):
+        # each view from the same start: the error of one backprojection
+        ctx.set_volume(0, x0)
+        corr = rng.standard_normal(det[0] * det[1]).astype(np.float32)
+        ctx.backproject_voxel(0, v, corr, 0.3)
+        got = ctx.get_volume(0)
+        ref = O.bp_voxel(g, q, corr, 0.3, gmax, x0)
+        ref64 = O.bp_voxel_f64(g, q, corr, 0.3, gmax, x0)
+        err, err64, o64 = rel_l2(got, ref), rel_l2(got, ref64), rel_l2(ref, ref64)
+        print(f"  view {v}: rel L2 {err:.2e} vs fp32 oracle, {err64:.2e} vs FP64 oracle "
+              f"(fp32 oracle vs FP64 {o64:.2e}); voxels changed {np.count_nonzero(got != x0)}",
+              flush=True)
+        # White-noise corrections at magnification up to 5 make a voxel's
+        # value sensitive to the fp32 rounding of its detector position: the
+        # bar is the fp32 oracle's own distance from FP64 (at most 1.5x it),
+        # and 2e-5 from the fp32 oracle.
+        assert err <= 2e-5, (v, err)
+        assert err64 <= 1.5 * o64 + 1e-6, (v, err64, o64)
+    ctx.close()
+
+
 def test_bp_ray_matches_oracle():
     print("[bp-ray] 20^3 det 32x32", flush=True)
     rng = np.random.default_rng(8)
