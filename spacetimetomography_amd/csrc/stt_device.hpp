@@ -27,8 +27,6 @@ struct alignas(256) ViewRec {
     // Plane marching of the projector: bits 0-1 dominant axis D of the view's
     // central ray (0 x, 1 y, 2 z), bit 2 set when that ray runs towards -D,
     // bit 3 set when the view samples the transposed (z-fastest) volume copy.
-    // Voxel BP: bit 4 set when rows 0 and 2 of MV*V2M have a zero y
-    // coefficient (detector column and depth constant along y).
     int march;
     double halfW, halfH, ds;
 };

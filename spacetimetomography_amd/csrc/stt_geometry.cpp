@@ -232,9 +232,6 @@ ViewRec build_view_from(const stt_geometry& g, const float view16[16], const flo
         }
     }
     rec.march = dom | (dsgn < 0 ? 4 : 0) | (dom == 0 ? 8 : 0);
-    // The voxel backprojector's y-column form: a voxel's detector column and
-    // depth do not depend on its y (every view rotated about the y axis)
-    if (rec.bp[1] == 0.0f && rec.bp[9] == 0.0f) rec.march |= 16;
     for (int c = 0; c < 3; ++c) rec.m2v[c] = m2v.e[5 * c];
     rec.m2v[3] = 1.0f;
     rec.halfW = halfW;

@@ -1256,24 +1256,15 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 // centre, bicubic (Catmull-Rom) sample of the correction image, x += alpha*v,
 // clamp [0, G_Max].  Atomic-free, one thread per voxel column segment.
 //
-// A 256-thread block owns a 32 x 8 x 16 voxel tile: lane = x within a
-// half-wave, a wave's two half-waves take two rows along the tile's third
-// axis T, and each thread walks 16 consecutive voxels along the column axis
-// C.  Two orientations:
-//   - y columns (C = y, T = z) for views whose projection keeps the detector
-//     column and the depth constant along y (ViewRec::march bit 4: rows 0
-//     and 2 of MV*V2M have a zero y coefficient, as for every view rotated
-//     about the y axis).  The reciprocal of the depth, the detector column
-//     and its four Catmull-Rom weights are computed once per column; along
-//     it only the detector row moves, by a constant step.
-//   - z columns (C = z, T = y) for any other view: the projected position
-//     moves along z by a fixed step of its three rows (one reciprocal per
-//     voxel).
-// The tile is a box, so the detector positions of its voxels lie inside the
-// bounding box of its eight corners' projections; the block stages that box
-// of the correction image in LDS with row loads and reads every voxel's 4x4
-// window from there: 16 taps per voxel from LDS instead of four unaligned
-// 16-byte global rows.
+// A 256-thread block owns a 32 x 8 x 16 voxel tile (a wave's two half-waves
+// take rows y0 + 2w and y0 + 2w + 1, lane = x within a half-wave, each
+// thread 16 consecutive z).  The tile is a box, so the detector positions of
+// its voxels lie inside the bounding box of its eight corners' projections;
+// the block stages that box of the correction image in LDS with row loads and
+// reads every voxel's 4x4 window from there: 16 taps per voxel from LDS
+// instead of four unaligned 16-byte global rows.  Stacking the tile along z
+// keeps the box small: it grows with the tile's x and y extent, not with
+// its 16 z steps.
 //
 // Blocks whose box lies at least two pixels inside the [2, W-2) x [2, H-2)
 // window and fits in LDS take a fast path: every voxel is updated and its
@@ -1281,17 +1272,13 @@ __device__ __noinline__ float2 exact_detector_position(const ViewRec& v, float f
 // corner projections).  Every other block takes the general path: per-voxel
 // skip test with the reference's own arithmetic near the window edges, and a
 // window read from global memory when it is not inside the staged box.
-constexpr int kBpTileX = 32;  // lanes of a half-wave (64 x 4 staged 22% fewer z-column blocks at 512^3 / 1024^2)
-constexpr int kBpTileT = 8;   // half-wave rows along the third axis
-constexpr int kBpVox = 16;    // voxels per thread along the column axis (8 and 32 measured slower for z columns)
+constexpr int kBpTileX = 32;  // 64 x 4 staged 22% fewer blocks at 512^3 / 1024^2 (the rest read global memory);
+constexpr int kBpTileY = 8;   // 16 x 16 was slower at both sizes
+constexpr int kBpVoxZ = 16;  // voxels per thread along z (8 and 32 measured slower)
 constexpr int kBpLdsFloats = 4096;  // 16 KB staged detector footprint per block (22 KB: the iteration 3% slower, as
                                     // fewer blocks share a CU with the other streams' projector blocks; 12 KB: more fall back)
 constexpr int kBpMaxCols = 192;     // three 64-lane column chunks
 constexpr int kBpMaxRows = 64;      // 16 rows per wave
-constexpr int kBpYPitch = 84;       // y columns: fixed LDS row pitch when the box fits (3 * 84 + 3 <= 255)
-constexpr int kViewYColumns = 16;   // ViewRec::march bit: u and depth constant along y
-constexpr int kBpLanesX = 8;        // x extent of a half-wave's patch: 8 x 4 voxel columns (32 x 1: 3.9 LDS bank-conflict cycles per read, 8 x 4: 0.9)
-constexpr bool kBpXcdSwizzle = true;
 
 // Catmull-Rom weights (Horner form of cr_weights' polynomials).
 __device__ __forceinline__ void cr_weights_h(float t, float w[4]) {
@@ -1302,76 +1289,34 @@ __device__ __forceinline__ void cr_weights_h(float t, float w[4]) {
     w[3] = t2 * fmaf(0.5f, t, -0.5f);
 }
 
-__device__ __forceinline__ float cr_dot(const float w[4], const float4& r) {
-    return fmaf(w[3], r.w, fmaf(w[2], r.z, fmaf(w[1], r.y, w[0] * r.x)));
-}
-
-__device__ __forceinline__ float bp_bicubic_w(const float wx[4], float dy, const float4 rows[4]) {
-    float wy[4];
+__device__ __forceinline__ float bp_bicubic(float dx, float dy, const float4 rows[4]) {
+    float wy[4], wx[4];
     cr_weights_h(dy, wy);
+    cr_weights_h(dx, wx);
     float h[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) h[b] = cr_dot(wx, rows[b]);
+    for (int b = 0; b < 4; ++b) h[b] = fmaf(wx[3], rows[b].w, fmaf(wx[2], rows[b].z, fmaf(wx[1], rows[b].y, wx[0] * rows[b].x)));
     return fmaf(wy[3], h[3], fmaf(wy[2], h[2], fmaf(wy[1], h[1], wy[0] * h[0])));
 }
 
-__device__ __forceinline__ float bp_bicubic(float dx, float dy, const float4 rows[4]) {
-    float wx[4];
-    cr_weights_h(dx, wx);
-    return bp_bicubic_w(wx, dy, rows);
-}
-
-// Catmull-Rom value at t of four samples in cubicInterpolate's Horner form
-// (ST_Tomography.cpp:183-186), fp32 with FMAs: 11 operations against 15
-// for the weights and their dot product.
-__device__ __forceinline__ float cr_poly(float t, float p0, float p1, float p2, float p3) {
-    const float a = p2 - p0;
-    const float b = fmaf(2.0f, p0, fmaf(-5.0f, p1, fmaf(4.0f, p2, -p3)));
-    const float c = fmaf(3.0f, p1 - p2, p3 - p0);
-    return fmaf(0.5f * t, fmaf(t, fmaf(t, c, b), a), p1);
-}
-
-// Interior loop of a y column: the detector column's weights wx are fixed,
-// the row moves by dpy per voxel.  P > 0: compile-time LDS row pitch (the
-// tap offsets become instruction immediates); P == 0: runtime pitch.
-template <int P>
-__device__ __forceinline__ void bp_ycol_interior(const float* win, int pitch_rt, unsigned cb, const float wx[4], float py0,
-                                                 float dpy, const PdConst& pc, float xs[kBpVox]) {
-    typedef __attribute__((address_space(3))) const float lds_cfloat;
-    typedef __attribute__((address_space(3))) const char lds_cchar;
-    const unsigned pitch = P > 0 ? (unsigned)P : (unsigned)pitch_rt;
-    lds_cchar* wb = (lds_cchar*)win + 4u * cb;  // 32-bit LDS byte addresses
-#pragma unroll
-    for (int i = 0; i < kBpVox; ++i) {
-        const float pyf = fmaf((float)i, dpy, py0);
-        const unsigned py = (unsigned)(int)pyf;
-        lds_cfloat* w0 = (lds_cfloat*)(wb + __umul24(py, 4u * pitch));
-        float h[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            h[b] = cr_dot(wx, make_float4(w0[b * pitch], w0[b * pitch + 1], w0[b * pitch + 2], w0[b * pitch + 3]));
-        const float gv = cr_poly(__builtin_amdgcn_fractf(pyf), h[0], h[1], h[2], h[3]);
-        xs[i] = __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xs[i]), 0.0f, pc.gmax);
-    }
-}
-
-// One tile of the backprojection; YC: y columns, else z columns (see above).
-template <bool YC>
-__device__ __forceinline__ void bp_tile(const WorkItem& it, const ViewRec& v, const VolGeom& g, const PdConst& pc,
-                                        float* win, int x0, int t0, int c0) {
+__global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ items, VolGeom g, PdConst pc) {
+    __shared__ float win[kBpLdsFloats];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nzt = (g.nz + kBpVoxZ - 1) / kBpVoxZ;
+    const int x0 = blockIdx.x * kBpTileX, y0 = blockIdx.y * kBpTileY;
+    const int z0 = (blockIdx.z % nzt) * kBpVoxZ;
+    const WorkItem& it = items[blockIdx.z / nzt];
+    const ViewRec& v = *it.view;
     const float* A = v.bp;
-    const int nT = YC ? g.nz : g.ny, nC = YC ? g.ny : g.nz;
-    const int xe = min(x0 + kBpTileX, g.nx) - 1, te = min(t0 + kBpTileT, nT) - 1, ce = min(c0 + kBpVox, nC) - 1;
+    const int xe = min(x0 + kBpTileX, g.nx) - 1, ye = min(y0 + kBpTileY, g.ny) - 1, ze = min(z0 + kBpVoxZ, g.nz) - 1;
 
     // Footprint: lanes 0..7 project one tile corner each.
     int lx, ly, hx, hy;
     {
         const float cx = (float)((lane & 1) ? xe : x0);
-        const float ct = (float)((lane & 2) ? te : t0);
-        const float cc = (float)((lane & 4) ? ce : c0);
-        const float cy = YC ? cc : ct, cz = YC ? ct : cc;
+        const float cy = (float)((lane & 2) ? ye : y0);
+        const float cz = (float)((lane & 4) ? ze : z0);
         const float ex = fmaf(A[0], cx, fmaf(A[1], cy, fmaf(A[2], cz, A[3])));
         const float ey = fmaf(A[4], cx, fmaf(A[5], cy, fmaf(A[6], cz, A[7])));
         const float ez = fmaf(A[8], cx, fmaf(A[9], cy, fmaf(A[10], cz, A[11])));
@@ -1398,121 +1343,92 @@ __device__ __forceinline__ void bp_tile(const WorkItem& it, const ViewRec& v, co
     const int bx0 = max(lx - 2, 1), by0 = max(ly - 2, 1);
     const int bx1 = min(hx + 3, g.W - 1), by1 = min(hy + 3, g.H - 1);
     const int cols = bx1 - bx0 + 1, rows = by1 - by0 + 1;
-    // LDS row pitch: y columns use kBpYPitch when the box fits it, so a
-    // voxel's 16 taps lie at immediate offsets from one address (ds_read2
-    // offsets up to 255 dwords: 3 * 84 + 3); otherwise the box's width.
-    const bool fixedp = YC && cols <= kBpYPitch && rows * kBpYPitch <= kBpLdsFloats;
-    const int pitch = fixedp ? kBpYPitch : cols;
-    const bool staged = cols > 0 && rows > 0 && cols <= kBpMaxCols && rows <= kBpMaxRows && pitch * rows <= kBpLdsFloats;
-    const bool full_tile = x0 + kBpTileX <= g.nx && t0 + kBpTileT <= nT && c0 + kBpVox <= nC;
+    const bool staged = cols > 0 && rows > 0 && cols <= kBpMaxCols && rows <= kBpMaxRows && cols * rows <= kBpLdsFloats;
+    const bool full_tile = x0 + kBpTileX <= g.nx && y0 + kBpTileY <= g.ny && z0 + kBpVoxZ <= g.nz;
     const bool interior = staged && full_tile && lx >= 4 && ly >= 4 && hx <= g.W - 5 && hy <= g.H - 5;
 
     // This thread's voxels are loaded before the footprint, so the two
     // latencies overlap.
-    // half-wave hw holds a kBpLanesX x (32 / kBpLanesX) patch of the tile's x-T face
-    constexpr int LX = kBpLanesX, GX = kBpTileX / LX;
-    const int hw = 2 * wv + (lane >> 5), l32 = lane & 31;
-    const int x = x0 + (hw % GX) * LX + (l32 % LX), t = t0 + (hw / GX) * (32 / LX) + l32 / LX;
-    const int n_here = (x < g.nx && t < nT) ? min(kBpVox, nC - c0) : 0;
-    const int y0 = YC ? c0 : t, z0 = YC ? t : c0;
+    const int x = x0 + (lane & 31), y = y0 + 2 * wv + (lane >> 5);
+    const int nz_here = (x < g.nx && y < g.ny) ? min(kBpVoxZ, g.nz - z0) : 0;
     const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)it.vol, (short)0, (int)g.vol_bytes, kBufWord3);
-    const unsigned vo = 4u * ((unsigned)x + (unsigned)g.nx * ((unsigned)y0 + (unsigned)g.ny * (unsigned)z0));
-    const unsigned step4 = YC ? 4u * (unsigned)g.nx : 4u * (unsigned)g.nx * (unsigned)g.ny;
-    float xs[kBpVox];
-    if (full_tile) {  // uniform: every lane has all kBpVox voxels
+    const int vo = 4 * (x + g.nx * y);
+    const int plane4 = 4 * g.nx * g.ny;
+    float xs[kBpVoxZ];
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i) xs[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rv, vo, i * step4, 0));
-    } else {
-#pragma unroll
-        for (int i = 0; i < kBpVox; ++i)
-            xs[i] = i < n_here ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rv, vo, i * step4, 0)) : 0.0f;
-    }
+    for (int i = 0; i < kBpVoxZ; ++i)
+        xs[i] = i < nz_here ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rv, vo, (z0 + i) * plane4, 0)) : 0.0f;
 
     const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void*)it.corr, (short)0, (int)g.det_bytes, kBufWord3);
     if (staged) {
-        // Rows wv, wv+4, ... of the box, up to 3 chunks of 64 columns each,
-        // with gfx950's direct-to-LDS buffer loads: lane l of a chunk lands
-        // at the chunk's LDS row offset + l (no register round trip, no LDS
-        // write instruction).  Lanes past the box's last column are masked
-        // off, so no chunk writes beyond its row.  Every address lies inside
-        // the image (rows by0..by1, columns bx0..bx1).
-        typedef __attribute__((address_space(3))) void lds_void;
-        typedef __attribute__((address_space(3))) float lds_float;
+        // Rows wv, wv+4, ... of the box; up to 16 rows x 3 column chunks per
+        // wave, all loads in flight before the first LDS write.  Lanes past
+        // the box's last column re-read that column (every address stays
+        // inside the image) and store nothing.
+        int co[3];
 #pragma unroll
-        for (int k = 0; k < kBpMaxRows / 4; ++k) {
+        for (int h = 0; h < 3; ++h) co[h] = 4 * min(lane + 64 * h, cols - 1);
+        float vals[16][3];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
             const int rr = wv + 4 * k;
-            if (rr >= rows) break;  // uniform
-            const unsigned so = 4u * (unsigned)((by0 + rr) * g.W + bx0);
+            if (rr < rows) {
+                const int so = 4 * ((by0 + rr) * g.W + bx0);
 #pragma unroll
-            for (int h = 0; h < kBpMaxCols / 64; ++h) {
-                if (64 * h >= cols) break;  // uniform
-                if (lane + 64 * h < cols)
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rc, (lds_void*)((lds_float*)win + (rr * pitch + 64 * h)), 4,
-                                                             4u * (unsigned)(lane + 64 * h), so, 0, 0);
+                for (int h = 0; h < 3; ++h)
+                    vals[k][h] = (64 * h < cols) ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rc, co[h], so, 0)) : 0.0f;
             }
         }
-        slab_stage_wait();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int rr = wv + 4 * k;
+            if (rr < rows) {
+#pragma unroll
+                for (int h = 0; h < 3; ++h)
+                    if (lane + 64 * h < cols) win[rr * cols + lane + 64 * h] = vals[k][h];
+            }
+        }
     }
     __syncthreads();
-    if (n_here == 0) return;
+    if (nz_here == 0) return;
 
-    const float fx = (float)x, fy = (float)y0, fz = (float)z0;
-    const float ex = fmaf(A[0], fx, fmaf(A[1], fy, fmaf(A[2], fz, A[3])));
-    const float ey = fmaf(A[4], fx, fmaf(A[5], fy, fmaf(A[6], fz, A[7])));
-    const float ez = fmaf(A[8], fx, fmaf(A[9], fy, fmaf(A[10], fz, A[11])));
-    // z columns: voxel i's rows are e + i * A[column z] (one rounding each,
-    // no drift along the column)
-    // y columns: depth and detector column fixed, the row steps by dpy
-    // (a correctly rounded reciprocal: one per column)
-    const float rc0 = YC ? 1.0f / ez : 0.0f;
-    const float pxc = YC ? fmaf(-ex, rc0, v.bp[12]) : 0.0f;
-    const float py0 = YC ? fmaf(-ey, rc0, v.bp[13]) : 0.0f;
-    const float dpy = YC ? -A[5] * rc0 : 0.0f;
+    const float fx = (float)x, fy = (float)y;
+    // voxel i's rows are e + i * A[column z]: one rounding each, no drift
+    // along the column
+    const float ex = fmaf(A[0], fx, fmaf(A[1], fy, fmaf(A[2], (float)z0, A[3])));
+    const float ey = fmaf(A[4], fx, fmaf(A[5], fy, fmaf(A[6], (float)z0, A[7])));
+    const float ez = fmaf(A[8], fx, fmaf(A[9], fy, fmaf(A[10], (float)z0, A[11])));
     bool any = false;
     if (interior) {
-        const int base = -(by0 + 1) * pitch - (bx0 + 1);
-        if (YC) {
-            const int px = (int)pxc;
-            float wx[4];
-            cr_weights_h(__builtin_amdgcn_fractf(pxc), wx);
-            const unsigned cb = (unsigned)(px + base);
-            if (fixedp) bp_ycol_interior<kBpYPitch>(win, kBpYPitch, cb, wx, py0, dpy, pc, xs);
-            else bp_ycol_interior<0>(win, pitch, cb, wx, py0, dpy, pc, xs);
-        } else {
+        const int base = -(by0 + 1) * cols - (bx0 + 1);
 #pragma unroll
-            for (int i = 0; i < kBpVox; ++i) {
-                const float r = __builtin_amdgcn_rcpf(fmaf(A[10], (float)i, ez));
-                const float pxf = fmaf(-fmaf(A[2], (float)i, ex), r, v.bp[12]);
-                const float pyf = fmaf(-fmaf(A[6], (float)i, ey), r, v.bp[13]);
-                const int px = (int)pxf, py = (int)pyf;
-                const float* w0 = win + (__mul24(py, pitch) + px + base);
-                float4 rw[4];
+        for (int i = 0; i < kBpVoxZ; ++i) {
+            const float r = __builtin_amdgcn_rcpf(fmaf(A[10], (float)i, ez));
+            const float pxf = fmaf(-fmaf(A[2], (float)i, ex), r, v.bp[12]);
+            const float pyf = fmaf(-fmaf(A[6], (float)i, ey), r, v.bp[13]);
+            const int px = (int)pxf, py = (int)pyf;
+            const float* w0 = win + (__mul24(py, cols) + px + base);
+            float4 rw[4];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) rw[b] = make_float4(w0[b * pitch], w0[b * pitch + 1], w0[b * pitch + 2], w0[b * pitch + 3]);
-                const float gv = bp_bicubic(__builtin_amdgcn_fractf(pxf), __builtin_amdgcn_fractf(pyf), rw);
-                xs[i] = __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xs[i]), 0.0f, pc.gmax);
-            }
+            for (int b = 0; b < 4; ++b) rw[b] = make_float4(w0[b * cols], w0[b * cols + 1], w0[b * cols + 2], w0[b * cols + 3]);
+            const float gv = bp_bicubic(__builtin_amdgcn_fractf(pxf), __builtin_amdgcn_fractf(pyf), rw);
+            xs[i] = __builtin_amdgcn_fmed3f(fmaf(pc.alpha, gv, xs[i]), 0.0f, pc.gmax);
         }
         any = true;
     } else {
         const float hxw = (float)(g.W - 2), hyw = (float)(g.H - 2);
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i) {
-            if (i < n_here) {
-                float pxf, pyf;
-                if (YC) {
-                    pxf = pxc;
-                    pyf = fmaf((float)i, dpy, py0);
-                } else {
-                    const float r = __builtin_amdgcn_rcpf(fmaf(A[10], (float)i, ez));
-                    pxf = fmaf(-fmaf(A[2], (float)i, ex), r, v.bp[12]);
-                    pyf = fmaf(-fmaf(A[6], (float)i, ey), r, v.bp[13]);
-                }
+        for (int i = 0; i < kBpVoxZ; ++i) {
+            if (i < nz_here) {
+                const float fz = (float)(z0 + i);
+                const float r = __builtin_amdgcn_rcpf(fmaf(A[10], (float)i, ez));
+                float pxf = fmaf(-fmaf(A[2], (float)i, ex), r, v.bp[12]);
+                float pyf = fmaf(-fmaf(A[6], (float)i, ey), r, v.bp[13]);
                 // Within 0.01 px of the window's edges the update/skip decision
                 // is taken with the reference's own arithmetic (rare).
                 if (fabsf(pxf - 2.0f) < 0.01f || fabsf(pxf - hxw) < 0.01f || fabsf(pyf - 2.0f) < 0.01f ||
                     fabsf(pyf - hyw) < 0.01f) {
-                    const float2 e = exact_detector_position(v, fx, YC ? (float)(y0 + i) : fy, YC ? fz : (float)(z0 + i));
+                    const float2 e = exact_detector_position(v, fx, fy, fz);
                     pxf = e.x;
                     pyf = e.y;
                 }
@@ -1522,10 +1438,10 @@ __device__ __forceinline__ void bp_tile(const WorkItem& it, const ViewRec& v, co
                     const int wx0 = px - 1 - bx0, wy0 = py - 1 - by0;
                     float4 rw[4];
                     if (staged && wx0 >= 0 && wx0 + 3 < cols && wy0 >= 0 && wy0 + 3 < rows) {
-                        const float* w0 = win + wy0 * pitch + wx0;
+                        const float* w0 = win + wy0 * cols + wx0;
 #pragma unroll
                         for (int b = 0; b < 4; ++b)
-                            rw[b] = make_float4(w0[b * pitch], w0[b * pitch + 1], w0[b * pitch + 2], w0[b * pitch + 3]);
+                            rw[b] = make_float4(w0[b * cols], w0[b * cols + 1], w0[b * cols + 2], w0[b * cols + 3]);
                     } else {
                         const int off = 4 * ((py - 1) * g.W + (px - 1));
 #pragma unroll
@@ -1540,57 +1456,17 @@ __device__ __forceinline__ void bp_tile(const WorkItem& it, const ViewRec& v, co
     }
     if (it.last) {
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i)
-            if (i < n_here) {
-                const int yi = YC ? y0 + i : y0, zi = YC ? z0 : z0 + i;
-                const size_t q = (size_t)x + (size_t)g.nx * ((size_t)yi + (size_t)g.ny * zi);
-                xs[i] = finalize_voxel(it, g, pc, x, yi, zi, q, xs[i]);
+        for (int i = 0; i < kBpVoxZ; ++i)
+            if (i < nz_here) {
+                const size_t q = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * (z0 + i));
+                xs[i] = finalize_voxel(it, g, pc, x, y, z0 + i, q, xs[i]);
             }
     } else if (!any) {
         return;
     }
-    if (full_tile) {
 #pragma unroll
-        for (int i = 0; i < kBpVox; ++i) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, xs[i]), rv, vo, i * step4, 0);
-    } else {
-#pragma unroll
-        for (int i = 0; i < kBpVox; ++i)
-            if (i < n_here) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, xs[i]), rv, vo, i * step4, 0);
-    }
-}
-
-// Tiles per work item: the larger of the two orientations' counts (equal for
-// cubic volumes); blocks past the count of their view's orientation exit.
-__host__ __device__ __forceinline__ int bp_tiles_per_item(const VolGeom& g) {
-    const int ntx = (g.nx + kBpTileX - 1) / kBpTileX;
-    const int zc = ((g.ny + kBpTileT - 1) / kBpTileT) * ((g.nz + kBpVox - 1) / kBpVox);
-    const int yc = ((g.nz + kBpTileT - 1) / kBpTileT) * ((g.ny + kBpVox - 1) / kBpVox);
-    return ntx * (zc > yc ? zc : yc);
-}
-
-// Grid (tiles, items).  Blocks are dealt to the 8 XCDs round-robin in
-// dispatch order; the remapping gives each XCD a contiguous eighth of the
-// (item, tile) range, so one XCD's L2 serves whole frames' correction images.
-__global__ __launch_bounds__(256) void k_bp_voxel(const WorkItem* __restrict__ items, VolGeom g, PdConst pc) {
-    __shared__ float win[kBpLdsFloats];
-    const int nb = bp_tiles_per_item(g);
-    const int n = nb * (int)gridDim.y;
-    int b = (int)blockIdx.x + nb * (int)blockIdx.y;
-    if (kBpXcdSwizzle && (n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
-    const int item = b / nb;
-    b -= item * nb;
-    const WorkItem& it = items[item];
-    const ViewRec& v = *it.view;
-    const bool yc = (v.march & kViewYColumns) != 0;
-    const int ntx = (g.nx + kBpTileX - 1) / kBpTileX;
-    const int nT = yc ? g.nz : g.ny, nC = yc ? g.ny : g.nz;
-    const int ntt = (nT + kBpTileT - 1) / kBpTileT, ntc = (nC + kBpVox - 1) / kBpVox;
-    if (b >= ntx * ntt * ntc) return;  // uniform per block
-    const int tx = b % ntx;
-    b /= ntx;
-    const int tt = b % ntt, tc = b / ntt;
-    if (yc) bp_tile<true>(it, v, g, pc, win, tx * kBpTileX, tt * kBpTileT, tc * kBpVox);
-    else bp_tile<false>(it, v, g, pc, win, tx * kBpTileX, tt * kBpTileT, tc * kBpVox);
+    for (int i = 0; i < kBpVoxZ; ++i)
+        if (i < nz_here) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, xs[i]), rv, vo, (z0 + i) * plane4, 0);
 }
 
 // Ray-BP set-up of one view, one block per work item, after the SART pass
@@ -2645,7 +2521,8 @@ hipError_t launch_transpose_xz(const WorkItem* items, int nitems, const VolGeom&
 }
 
 hipError_t launch_bp_voxel(const WorkItem* items, int nitems, const VolGeom& g, const PdConst& pc, hipStream_t s) {
-    const dim3 grid(bp_tiles_per_item(g), nitems);
+    const dim3 grid((g.nx + kBpTileX - 1) / kBpTileX, (g.ny + kBpTileY - 1) / kBpTileY,
+                    ((g.nz + kBpVoxZ - 1) / kBpVoxZ) * nitems);
     hipLaunchKernelGGL(k_bp_voxel, grid, dim3(256), 0, s, items, g, pc);
     return hipGetLastError();
 }

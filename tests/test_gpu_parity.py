@@ -165,9 +165,10 @@ def _quat_axis(axis, deg):
     return np.array([*(a * np.sin(h)), np.cos(h)], np.float32)
 
 
-# (vol, det, sid, sdd): magnification about 1.5-2 (y-column boxes fit the
-# fixed LDS pitch), about 3 (runtime pitch), about 4-5 (boxes too tall for LDS:
-# the general path's global window reads)
+# (vol, det, sid, sdd): magnification about 1.5-2 (most tiles take the
+# interior path), about 3 (wider staged windows, partly past the detector's
+# edges), about 4-5 (windows too large for LDS: the general path's global
+# window reads)
 BP_TILE_CASES = [
     ((64, 64, 64), (136, 136), 300.0, 500.0),
     ((72, 48, 80), (220, 150), 180.0, 500.0),
@@ -177,9 +178,9 @@ BP_TILE_CASES = [
 
 @pytest.mark.parametrize("case", range(len(BP_TILE_CASES)))
 def test_bp_voxel_tiles_match_oracle(case):
-    """Both backprojector orientations against the oracle on volumes with
-    interior tiles: views about the y axis take the y-column form, tilted
-    views (a quaternion about an oblique axis) the z-column form."""
+    """The voxel backprojector against the oracle on volumes with interior
+    tiles, at three magnifications, for views about the y axis and views
+    about two oblique axes (quaternions), one view at a time."""
     vol, det, sid, sdd = BP_TILE_CASES[case]
     print(f"[bp-tiles] case={case} vol={vol} det={det} sid={sid} sdd={sdd}", flush=True)
     rng = np.random.default_rng(70 + case)
