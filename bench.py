@@ -48,6 +48,16 @@ CONFIGS = {
 }
 
 
+def projector_kernel_name(tile):
+    """Kernel of a projector shape (stt_set_projector_tile): 0, -32, -8 are
+    the slab projector with 16x16, 32x8 and 8x32 rays per block; positive
+    values the level-synchronous projector with that many rays per wave row."""
+    if tile <= 0:
+        tw = 16 if tile == 0 else -tile
+        return f"k_fp_slab<SART,{tw},{256 // tw}>"
+    return f"k_fp_level<SART,{tile}>"
+
+
 def views_of(cfg, f):
     P = cfg["P"]
     return P[f] if isinstance(P, list) else P
@@ -630,7 +640,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": (f"{'k_fp_slab<SART>' if tile == 0 else f'k_fp_level<SART,{tile}>'} "
+            "kernel": (f"{projector_kernel_name(tile)} "
                        "(forward projection + fused SART correction, all frames' view k)"),
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
