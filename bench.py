@@ -176,17 +176,20 @@ def copy_calibration(mib):
     log(f"copy calibration: {mib} MiB device-to-device")
 
 
-def load_traffic(config, shape):
+def load_traffic(config, shape, box_kib):
     """HBM traffic per projector launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated against a copy kernel),
-    or None when no summary matches this configuration."""
+    or None when no summary matches this configuration (projector shape and,
+    for the slab shapes, the staged box size)."""
     path = os.path.join(ROOT, "profiles", f"traffic_projector_{config}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t if t.get("shape") == shape else None
+    if t.get("shape") != shape or (shape <= 0 and t.get("box_kib", 24) != box_kib):
+        return None
+    return t
 
 
 def cpu_baseline(g, p, cfg, angles, projections, nviews, faithful):
@@ -512,6 +515,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--projector-tile", type=int, default=None,
                     help="projector shape (0, -32, -8 slab; 64/32/16/8 level-synchronous); default: autotune")
+    ap.add_argument("--projector-box", type=int, default=None, choices=[16, 20, 24, 28],
+                    help="slab projector's staged box in KiB; default: autotune with a slab shape, else 24")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (counter calibration)")
     ap.add_argument("--flow", action="store_true",
@@ -558,6 +563,15 @@ def main():
         tile, tile_ms = ctx.autotune()
         log(f"autotune: projector tile {tile} (ms per batched SART launch, shapes 0/-32/-8/64/32/16/8: "
             f"{[round(v, 4) for v in tile_ms]})")
+    box = 24
+    if args.projector_box is not None:
+        box = args.projector_box
+        ctx.set_projector_box(box)
+        log(f"projector box {box} KiB (fixed)")
+    elif tile <= 0:
+        box, box_ms = ctx.autotune_box()
+        log(f"autotune: projector box {box} KiB (ms per batched SART launch, boxes 16/20/24/28 KiB: "
+            f"{[round(v, 4) for v in box_ms]})")
     batched = args.form != "plain"
     use_graph = args.form == "batched-graph"
 
@@ -615,7 +629,7 @@ def main():
     proj_iter_bytes = sum(Ps) * (4 * V + 8 * D)  # every frame-view's projector bytes
 
     views_desc = str(Ps[0]) if len(set(Ps)) == 1 else f"{min(Ps)}-{max(Ps)} (seeded, total {sum(Ps)})"
-    traffic = load_traffic(args.config, tile) if batched else None
+    traffic = load_traffic(args.config, tile, box) if batched else None
     out = {
         "metric": METRIC if args.config == "st256" else
         "ST recon iters/sec at %dx%dx%d x %d frames; projector %% HBM peak" % (*cfg["vol"], cfg["F"]),
@@ -637,6 +651,7 @@ def main():
             "form": args.form,
             "volume": list(vol), "frames": F, "views_per_frame": cfg["P"], "detector": list(det),
             "projector_tile": tile,
+            "projector_box_kib": box if tile <= 0 else None,
         },
         "roofline": {
             "bound": "hbm",

@@ -277,6 +277,21 @@ int  stt_set_projector_tile(stt_ctx* ctx, int rays_per_row);
  * the choice and ms_per_shape[STT_PROJECTOR_SHAPES] (may be NULL) the time
  * per view index of shapes 0/-32/-8/64/32/16/8. */
 int  stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape);
+/* Staged box ("brick") of the slab projector shapes, in KiB of LDS per
+ * block: 16, 20, 24 (default) or 28.  A larger box lets a slab span more
+ * planes (fewer staging rounds per ray) and leaves fewer blocks per CU.  The
+ * box only decides where a tap is read from (LDS or global memory), never
+ * its value, so projections are bit-identical for every size.  No
+ * counterpart in the reference (its projector reads the volume directly);
+ * the context's captured graph is rebuilt. */
+#define STT_PROJECTOR_BOXES 4
+int  stt_set_projector_box(stt_ctx* ctx, int kib);
+/* Times the box sizes 16/20/24/28 KiB with the context's current slab shape
+ * on the same launches as stt_autotune and keeps the fastest; *kib (may be
+ * NULL) receives it and ms_per_box[STT_PROJECTOR_BOXES] (may be NULL) the
+ * time per view index of each size.  STT_ERR_INVALID while a
+ * level-synchronous shape is selected. */
+int  stt_autotune_box(stt_ctx* ctx, int* kib, double* ms_per_box);
 
 /* Measurement helper: the projector launches of one iteration (every
  * frame's views, layout copies included, SART mode) timed with HIP events,

@@ -12,4 +12,5 @@ from ._lib import (  # noqa: F401
     STT_BP_VOXEL, STT_BP_RAY, STT_PRIOR_ATV, STT_PRIOR_TV, STT_PRIOR_STV,
     STT_INTERP_CUBIC_CLAMPED, STT_INTERP_TRILINEAR,
     STT_BUF_XBAR, STT_BUF_XPREV, STT_BUF_YTV, STT_BUF_YTP, STT_BUF_YBC, STT_BUF_FLOW,
+    PROJECTOR_SHAPES, PROJECTOR_BOXES,
 )

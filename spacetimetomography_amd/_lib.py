@@ -25,6 +25,7 @@ STT_INTERP_CUBIC_CLAMPED, STT_INTERP_TRILINEAR = 0, 1
 STT_BUF_XBAR, STT_BUF_XPREV, STT_BUF_YTV, STT_BUF_YTP, STT_BUF_YBC, STT_BUF_FLOW = range(6)
 PROJECTOR_SHAPES = (0, -32, -8, 64, 32, 16, 8)  # stt_autotune order (STT_PROJECTOR_SHAPES)
 PROJECTOR_SHAPES_N = len(PROJECTOR_SHAPES)
+PROJECTOR_BOXES = (16, 20, 24, 28)  # stt_autotune_box order (STT_PROJECTOR_BOXES), KiB
 
 
 class Geometry(C.Structure):
@@ -125,6 +126,8 @@ _SIGS = {
     "stt_time_projector": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "stt_set_projector_tile": (C.c_int, [_P, C.c_int]),
     "stt_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "stt_set_projector_box": (C.c_int, [_P, C.c_int]),
+    "stt_autotune_box": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "stt_residual_norm": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "stt_residual_norm_noalloc": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "stt_power_iteration": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
@@ -526,6 +529,15 @@ class Context:
         best = C.c_int()
         ms = (C.c_double * PROJECTOR_SHAPES_N)()
         check(lib.stt_autotune(self.h, C.byref(best), ms))
+        return best.value, list(ms)
+
+    def set_projector_box(self, kib):
+        check(lib.stt_set_projector_box(self.h, kib))
+
+    def autotune_box(self):
+        best = C.c_int()
+        ms = (C.c_double * len(PROJECTOR_BOXES))()
+        check(lib.stt_autotune_box(self.h, C.byref(best), ms))
         return best.value, list(ms)
 
 

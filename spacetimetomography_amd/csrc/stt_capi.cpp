@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/stt.h"
@@ -1011,6 +1012,7 @@ int stt_ctx_create(const stt_geometry* g, const stt_params* p, int nframes, cons
         c->vg.box_eps = 1.0f / 32.0f + 0.5f * ulp * (float)c->vg.max_samples;
     }
     c->vg.acc_wshift = raybp_weight_shift(c->vg);
+    c->vg.fp_box_kib = 24;
     c->vg.vol_bytes = (unsigned)(c->Vpad * sizeof(float));
     c->vg.volT_bytes = (unsigned)(c->VpadT * sizeof(float));
     c->vg.det_bytes = (unsigned)((size_t)g->det[0] * g->det[1] * sizeof(float));
@@ -1781,17 +1783,17 @@ int stt_set_projector_tile(stt_ctx* ctx, int rays_per_row) {
     return STT_OK;
 }
 
-int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
-    CTX_CHECK(ctx);
-    int rc = ctx->ready_check();
-    if (rc) return rc;
-    const int shapes[STT_PROJECTOR_SHAPES] = {0, -32, -8, 64, 32, 16, 8};
+namespace {
+
+// Times the projector launches a SART sweep issues (for the first view
+// indices k: the layout copy and the SART projector over every frame's view
+// k, batched form) under each of n configurations that set(i) applies to the
+// context's VolGeom, and returns the fastest index.  Corrections and ray
+// lengths go to scratch buffers, so no solver state changes (the layout
+// copy is rebuilt before every use).  ms[i] (may be NULL) receives the time
+// per view index.
+int time_projector_configs(stt_ctx* ctx, int n, const std::function<void(int)>& set, double* ms, int* best_index) {
     const size_t D = ctx->D;
-    if ((rc = ctx->ensure_transposed())) return rc;  // the level shapes under test need the copies
-    // The launches a sweep issues: for view index k, the layout copy and the
-    // SART projector over every frame's view k (batched form), here for the
-    // first few k.  Corrections and ray lengths go to scratch buffers, so no
-    // solver state changes (the layout copy is rebuilt before every use).
     const int nk = std::min(ctx->maxP, 4);
     std::vector<WorkItem> items;
     std::vector<int> off, cnt;
@@ -1801,25 +1803,19 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
             if (k < ctx->P[f]) items.push_back(ctx->make_item(f, k, false));
         cnt.push_back((int)items.size() - off.back());
     }
-    float* buf = nullptr;
     StreamMem buf_mem;
     HIPCHK(buf_mem.alloc(2 * items.size() * D * sizeof(float), ctx->stream));
-    buf = (float*)buf_mem.p;
-    for (size_t n = 0; n < items.size(); ++n) {
-        items[n].corr = buf + 2 * n * D;
-        items[n].raylen = buf + (2 * n + 1) * D;
+    float* buf = (float*)buf_mem.p;
+    for (size_t i = 0; i < items.size(); ++i) {
+        items[i].corr = buf + 2 * i * D;
+        items[i].raylen = buf + (2 * i + 1) * D;
     }
     HIPCHK(hipMemsetAsync(buf, 0, 2 * items.size() * D * sizeof(float), ctx->stream));
-    rc = ctx->upload_scratch(items);
-    if (rc) {
-        return rc;
-    }
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    const int old = ctx->vg.fp_tile;
-    int best = old;
-    double best_ms = 1e30;
+    int rc = ctx->upload_scratch(items);
+    if (rc) return rc;
+    OwnedEvent e0, e1;
+    HIPCHK(hipEventCreate(&e0.e));
+    HIPCHK(hipEventCreate(&e1.e));
     auto sweep = [&]() -> hipError_t {
         for (int k = 0; k < nk; ++k) {
             const WorkItem* it = ctx->items_scratch + off[k];
@@ -1829,29 +1825,81 @@ int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
         }
         return hipSuccess;
     };
-    for (int si = 0; si < STT_PROJECTOR_SHAPES; ++si) {
-        ctx->vg.fp_tile = shapes[si];
+    double best_ms = 1e30;
+    *best_index = 0;
+    for (int i = 0; i < n; ++i) {
+        set(i);
         HIPCHK(sweep());  // warm-up
-        HIPCHK(hipEventRecord(e0, ctx->stream));
+        HIPCHK(hipEventRecord(e0.e, ctx->stream));
         for (int rep = 0; rep < 2; ++rep) HIPCHK(sweep());
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        const double per = ms / (2.0 * nk);
-        if (ms_per_shape) ms_per_shape[si] = per;
+        HIPCHK(hipEventRecord(e1.e, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1.e));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0.e, e1.e));
+        const double per = t / (2.0 * nk);
+        if (ms) ms[i] = per;
         if (per < best_ms) {
             best_ms = per;
-            best = shapes[si];
+            *best_index = i;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return STT_OK;
+}
+
+const int kProjectorBoxKib[STT_PROJECTOR_BOXES] = {16, 20, 24, 28};
+
+}  // namespace
+
+int stt_autotune(stt_ctx* ctx, int* rays_per_row, double* ms_per_shape) {
+    CTX_CHECK(ctx);
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    const int shapes[STT_PROJECTOR_SHAPES] = {0, -32, -8, 64, 32, 16, 8};
+    if ((rc = ctx->ensure_transposed())) return rc;  // the level shapes under test need the copies
+    const int old = ctx->vg.fp_tile;
+    int bi = 0;
+    rc = time_projector_configs(ctx, STT_PROJECTOR_SHAPES, [&](int i) { ctx->vg.fp_tile = shapes[i]; }, ms_per_shape,
+                                &bi);
+    if (rc) {
+        ctx->vg.fp_tile = old;
+        return rc;
+    }
+    const int best = shapes[bi];
     ctx->vg.fp_tile = best;
     if (best != old) ctx->invalidate_graph();
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     if (best <= 0 && (rc = ctx->drop_transposed())) return rc;
     if (rays_per_row) *rays_per_row = best;
+    return STT_OK;
+}
+
+int stt_set_projector_box(stt_ctx* ctx, int kib) {
+    CTX_CHECK(ctx);
+    if (kib != 16 && kib != 20 && kib != 24 && kib != 28)
+        return fail(STT_ERR_INVALID, "projector box must be 16, 20, 24 or 28 KiB");
+    if (ctx->vg.fp_box_kib != kib) {
+        ctx->vg.fp_box_kib = kib;
+        ctx->invalidate_graph();
+    }
+    return STT_OK;
+}
+
+int stt_autotune_box(stt_ctx* ctx, int* kib, double* ms_per_box) {
+    CTX_CHECK(ctx);
+    int rc = ctx->ready_check();
+    if (rc) return rc;
+    if (ctx->vg.fp_tile > 0) return fail(STT_ERR_INVALID, "box sizes apply to the slab projector shapes (0, -32, -8)");
+    const int old = ctx->vg.fp_box_kib;
+    int bi = 0;
+    rc = time_projector_configs(ctx, STT_PROJECTOR_BOXES, [&](int i) { ctx->vg.fp_box_kib = kProjectorBoxKib[i]; },
+                                ms_per_box, &bi);
+    if (rc) {
+        ctx->vg.fp_box_kib = old;
+        return rc;
+    }
+    ctx->vg.fp_box_kib = kProjectorBoxKib[bi];
+    if (ctx->vg.fp_box_kib != old) ctx->invalidate_graph();
+    if (kib) *kib = ctx->vg.fp_box_kib;
     return STT_OK;
 }
 

@@ -48,6 +48,7 @@ struct VolGeom {
     unsigned vol_bytes;  // bytes of one padded volume allocation (buffer range)
     unsigned volT_bytes; // bytes of one padded transposed (z-fastest) volume
     unsigned det_bytes;  // bytes of one detector image
+    int fp_box_kib;      // slab projector's staged box per block: 16, 20, 24 (default) or 28 KiB of LDS
 };
 
 // Solver constants (reference globals ST_Tomography.cpp:88-111, 219).

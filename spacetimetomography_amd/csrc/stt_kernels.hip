@@ -703,7 +703,8 @@ __global__ __launch_bounds__(256) void k_fp_level(const WorkItem* __restrict__ i
 // samples of grazing rays --, off the box by rounding, or a slab too wide for
 // LDS) reads them from global memory.  No layout copy of the volume is needed
 // for any view direction.
-constexpr int kSlabCap = 6144;  // floats of staged box per block (24 KB; 20 and 28 KB measured slower)
+constexpr int kSlabCap = 6144;  // default floats of staged box per block (24 KB); the box size is a template
+                                // argument (16, 20, 24 or 28 KB: stt_set_projector_box / stt_autotune_box)
 constexpr int kSlabK = 23;      // most sample planes per slab (box depth <= 24)
 constexpr int kSlabKMin = 8;    // fewest sample planes per slab (before the LDS cap)
 constexpr float kSlabShear = 0.7f;  // slab depth: the box may widen by this fraction of its near-plane width
@@ -785,8 +786,9 @@ __device__ __forceinline__ void block_minmax(float v[NV], const bool is_max[NV],
     }
 }
 
-// Box of the slab whose sample base planes start at c0 (uniform arithmetic).
-template <int D, bool DOWN>
+// Box of the slab whose sample base planes start at c0 (uniform arithmetic);
+// CAP floats of LDS hold it.
+template <int D, bool DOWN, int CAP>
 __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const float o[3], float sAmin, float sAmax,
                                             float sBmin, float sBmax, float dfirst) {
     constexpr int A = D == 0 ? 1 : 0;
@@ -834,9 +836,9 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
         // thinner slab when the box would not fit; its true box only shrinks,
         // so the extents above stay a valid (conservative) bound
         const int per_plane = (nA | 1) * nB;
-        if (per_plane * (K + 1) > kSlabCap) {
-            K = (int)((float)kSlabCap * __builtin_amdgcn_rcpf((float)per_plane)) - 1;
-            while (K > 0 && per_plane * (K + 1) > kSlabCap) --K;
+        if (per_plane * (K + 1) > CAP) {
+            K = (int)((float)CAP * __builtin_amdgcn_rcpf((float)per_plane)) - 1;
+            while (K > 0 && per_plane * (K + 1) > CAP) --K;
             K = max(K, 1);
         }
     }
@@ -861,7 +863,7 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     bx.n[B] = nB;
     bx.px = bx.n[0] | 1;
     bx.pxy = (bx.px * bx.n[1]) | 1;
-    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= kSlabCap &&
+    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= CAP &&
                 bx.px * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
     bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
     // Inner slab: the sampler's clamp to [0, dims-1] is the identity on all
@@ -980,7 +982,7 @@ __device__ __forceinline__ void slab_walk(RayGen& G, const VolGeom& g, const Sla
 
 // COUNT: the sample count is an output (FP mode); the SART pass needs only
 // the line integral, so its unchecked loop keeps no count.
-template <int D, bool DOWN, int NT, bool COUNT>
+template <int D, bool DOWN, int NT, bool COUNT, int CAP>
 __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
                                           float* box, float* red, int* slots) {
     constexpr int A = D == 0 ? 1 : 0;
@@ -1063,7 +1065,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (any_ray) {
         if (wv == 0) {
-            SlabBox b0 = slab_box<D, DOWN>(g, c0, o, q[0], q[1], q[2], q[3], q[5]);
+            SlabBox b0 = slab_box<D, DOWN, CAP>(g, c0, o, q[0], q[1], q[2], q[3], q[5]);
             if (!slopes_ok) b0.staged = false;
             box_put(slots, b0);
         }
@@ -1074,7 +1076,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         const SlabBox bx = box_get<DOWN>(slots + kBoxInts * (guard & 1));
         if (bx.staged) slab_stage<NT>(vv, g, bx, box);
         if (wv == (guard & (NT / 64 - 1))) {
-            SlabBox nb = slab_box<D, DOWN>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);
+            SlabBox nb = slab_box<D, DOWN, CAP>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);
             if (!slopes_ok) nb.staged = false;
             box_put(slots + kBoxInts * ((guard + 1) & 1), nb);
         }
@@ -1151,10 +1153,10 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
     return density;
 }
 
-template <int MODE, int TW, int TH>
+template <int MODE, int TW, int TH, int CAP>
 __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__ items, VolGeom g, float factor) {
     constexpr int NT = TW * TH;
-    __shared__ float box[kSlabCap];
+    __shared__ float box[CAP];
     __shared__ float red[6 * NT / 64];
     __shared__ int slots[2 * kBoxInts];
     int i, j;
@@ -1169,12 +1171,12 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     int ns = 0;
     float value;
     switch (march) {
-        case 0: value = slab_ray<0, false, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
-        case 4: value = slab_ray<0, true, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
-        case 1: value = slab_ray<1, false, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
-        case 5: value = slab_ray<1, true, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
-        case 6: value = slab_ray<2, true, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
-        default: value = slab_ray<2, false, NT, MODE == FP_MODE_FP>(it, g, r, hit, ns, box, red, slots); break;
+        case 0: value = slab_ray<0, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 4: value = slab_ray<0, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 1: value = slab_ray<1, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 5: value = slab_ray<1, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 6: value = slab_ray<2, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        default: value = slab_ray<2, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
     }
     if (!valid) return;
     const int k = i + j * g.W;
@@ -2469,15 +2471,26 @@ static inline dim3 vol_grid(const VolGeom& g, int nitems) {
 // per 256-thread block, transposed copy for x-dominant views in the SART
 // sweep).
 
+template <int MODE, int CAP>
+static void launch_fp_slab_cap(int tw, dim3 grid, const WorkItem* items, const VolGeom& g, float factor, hipStream_t s) {
+    switch (tw) {
+        case 32: hipLaunchKernelGGL((k_fp_slab<MODE, 32, 8, CAP>), grid, dim3(256), 0, s, items, g, factor); break;
+        case 8: hipLaunchKernelGGL((k_fp_slab<MODE, 8, 32, CAP>), grid, dim3(256), 0, s, items, g, factor); break;
+        default: hipLaunchKernelGGL((k_fp_slab<MODE, 16, 16, CAP>), grid, dim3(256), 0, s, items, g, factor); break;
+    }
+}
+
 template <int MODE>
 static hipError_t launch_fp_slab(const WorkItem* items, int nitems, const VolGeom& g, float factor, hipStream_t s) {
-    // fp_tile 0: 16 x 16 rays per block; -32: 32 x 8; -8: 8 x 32
+    // fp_tile 0: 16 x 16 rays per block; -32: 32 x 8; -8: 8 x 32.  fp_box_kib:
+    // the staged box (16, 20, 24 or 28 KiB of LDS per block)
     const int tw = g.fp_tile == 0 ? 16 : -g.fp_tile, th = 256 / tw;
     const dim3 grid(((g.W + tw - 1) / tw) * ((g.H + th - 1) / th), nitems);
-    switch (tw) {
-        case 32: hipLaunchKernelGGL((k_fp_slab<MODE, 32, 8>), grid, dim3(256), 0, s, items, g, factor); break;
-        case 8: hipLaunchKernelGGL((k_fp_slab<MODE, 8, 32>), grid, dim3(256), 0, s, items, g, factor); break;
-        default: hipLaunchKernelGGL((k_fp_slab<MODE, 16, 16>), grid, dim3(256), 0, s, items, g, factor); break;
+    switch (g.fp_box_kib) {
+        case 16: launch_fp_slab_cap<MODE, 4096>(tw, grid, items, g, factor, s); break;
+        case 20: launch_fp_slab_cap<MODE, 5120>(tw, grid, items, g, factor, s); break;
+        case 28: launch_fp_slab_cap<MODE, 7168>(tw, grid, items, g, factor, s); break;
+        default: launch_fp_slab_cap<MODE, kSlabCap>(tw, grid, items, g, factor, s); break;
     }
     return hipGetLastError();
 }

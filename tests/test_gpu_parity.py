@@ -124,6 +124,57 @@ def test_slab_equals_level_on_fine_detector():
         ctx.close()
 
 
+def test_projector_box_sizes_bit_identical():
+    """The slab projector's staged box size (16/20/24/28 KiB) only decides
+    where a tap is read from, LDS or global memory: for every slab tile the
+    forward projections (values, sample counts, ray lengths) and a SART
+    sweep from the same start are bit-identical across sizes.  The box
+    autotuner keeps one of the sizes and refuses a level-synchronous shape."""
+    dims, det = (96, 96, 96), (192, 192)
+    print(f"[fp-box] {dims}, {det}, boxes {stt.PROJECTOR_BOXES}", flush=True)
+    g = stt.Geometry.make(dims, det, (1, 1, 1), 1.0, 300.0, 500.0)
+    angles = [0.0, 30.0, 44.99, 45.0, 91.0, 200.0]
+    p = stt.params_default()
+    p.alpha = 0.3
+    ctx = stt.Context(g, p, [len(angles)])
+    ctx.set_views(0, angles)
+    x0 = stt.synth_phantom_frame(dims, 0, n_ellipsoids=6, seed=5, deform_seed=2, amp=2.0)
+    ctx.set_volume(0, x0)
+    b = np.concatenate([ctx.forward_project(0, k)[0] for k in range(len(angles))])
+    ctx.set_projections(0, b * 1.05)
+    try:
+        for tile in (0, -32, -8):
+            ctx.set_projector_tile(tile)
+            ref = None
+            for kib in stt.PROJECTOR_BOXES:
+                ctx.set_projector_box(kib)
+                ctx.set_volume(0, x0)
+                out = [ctx.forward_project(0, k) for k in range(len(angles))]
+                ctx.sart_sweep(0)
+                swept = ctx.get_volume(0)
+                if ref is None:
+                    ref = (out, swept)
+                    continue
+                for k in range(len(angles)):
+                    for a, r in zip(out[k], ref[0][k]):
+                        assert np.array_equal(a, r), (tile, kib, angles[k])
+                assert np.array_equal(swept, ref[1]), (tile, kib)
+            print(f"  tile {tile}: projections and sweep identical across boxes", flush=True)
+        ctx.set_projector_tile(0)
+        kib, ms = ctx.autotune_box()
+        print(f"  autotune_box -> {kib} KiB, ms per view index {[round(v, 4) for v in ms]}", flush=True)
+        assert kib in stt.PROJECTOR_BOXES and all(v > 0 for v in ms)
+        ctx.set_projector_tile(16)
+        with pytest.raises(stt.SttError):
+            ctx.autotune_box()
+        with pytest.raises(stt.SttError):
+            ctx.set_projector_box(32)
+    finally:
+        ctx.set_projector_tile(0)
+        ctx.set_projector_box(24)
+        ctx.close()
+
+
 def test_fp_constant_volume_closed_form():
     """Constant volume c: every hit ray integrates to c * sd * (number of samples)."""
     print("[fp-const] 32^3, 48x48", flush=True)
