@@ -63,12 +63,12 @@ def views_of(cfg, f):
     return P[f] if isinstance(P, list) else P
 
 
-def setup(stt, cfg):
+def setup(stt, cfg, bp="voxel"):
     vol, det, F = cfg["vol"], cfg["det"], cfg["F"]
     g = stt.Geometry.make(vol, det, (1.0, 1.0, 1.0), 1.0, 600.0, 1000.0)
     p = stt.params_default()
     p.sigma, p.tau, p.lambda_tv, p.k_tp, p.e_bc, p.huber, p.alpha = 0.2, 0.2, 0.01, 0.1, 0.05, 0.01, 0.3
-    p.bp = stt.STT_BP_VOXEL
+    p.bp = stt.STT_BP_RAY if bp == "ray" else stt.STT_BP_VOXEL
     p.prior = stt.STT_PRIOR_STV
     p.interp = stt.STT_INTERP_CUBIC_CLAMPED
     Ps = [views_of(cfg, f) for f in range(F)]
@@ -515,6 +515,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--projector-tile", type=int, default=None,
                     help="projector shape (0, -32, -8 slab; 64/32/16/8 level-synchronous); default: autotune")
+    ap.add_argument("--bp", default="voxel", choices=["voxel", "ray"],
+                    help="backprojector of the SART update (the reference's --bp flag); the headline is voxel")
     ap.add_argument("--projector-box", type=int, default=None, choices=[16, 20, 24, 28],
                     help="slab projector's staged box in KiB; default: autotune with a slab shape, else 24")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -553,7 +555,7 @@ def main():
 
     cfg = CONFIGS[args.config]
     t0 = time.time()
-    g, p, ctx, angles, projections = setup(stt, cfg)
+    g, p, ctx, angles, projections = setup(stt, cfg, args.bp)
     log(f"setup done in {time.time() - t0:.1f}s")
     if args.projector_tile is not None:
         tile = args.projector_tile
@@ -631,8 +633,9 @@ def main():
     views_desc = str(Ps[0]) if len(set(Ps)) == 1 else f"{min(Ps)}-{max(Ps)} (seeded, total {sum(Ps)})"
     traffic = load_traffic(args.config, tile, box) if batched else None
     out = {
-        "metric": METRIC if args.config == "st256" else
-        "ST recon iters/sec at %dx%dx%d x %d frames; projector %% HBM peak" % (*cfg["vol"], cfg["F"]),
+        "metric": (METRIC if args.config == "st256" else
+                   "ST recon iters/sec at %dx%dx%d x %d frames; projector %% HBM peak" % (*cfg["vol"], cfg["F"]))
+        + ("" if args.bp == "voxel" else "; ray-driven BP"),
         "value": args.steps / elapsed,
         "unit": "iters/s",
         "n_gpus": 1,
@@ -647,7 +650,7 @@ def main():
         "config": {
             "workload": (f"space-time PD+SART reconstruction {vol[0]}x{vol[1]}x{vol[2]} fp32 x {F} frames, "
                          f"{views_desc} {['even', 'golden-angle', 'van der Corput'][cfg['angles']]} views/frame, "
-                         f"{det[0]}x{det[1]} detector, STV prior, voxel BP, clamped-cubic warp"),
+                         f"{det[0]}x{det[1]} detector, STV prior, {args.bp} BP, clamped-cubic warp"),
             "form": args.form,
             "volume": list(vol), "frames": F, "views_per_frame": cfg["P"], "detector": list(det),
             "projector_tile": tile,

@@ -1201,6 +1201,244 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Ray-driven BP scatter through the slab boxes (the BP-mode pass of
+// GenerateImage_MT, ST_Tomography.cpp:733-739; per sample and tap
+// SingleScalarVolume.core.h:73-84,128-143: contribution w*corr and weight w).
+// The addends are SinkFixed's exact 64-bit fixed-point values.  A block
+// marches its 16x16 rays through the same slabs as the slab projector and
+// adds each slab's addends into an LDS copy of the slab box with 64-bit LDS
+// atomics; after the slab every non-zero box voxel is added to the global
+// sums with one atomic per sum.  Integer addition is associative, so the
+// sums equal those of one global atomic per sample and tap (k_fp<RAYBP>) bit
+// for bit, with about a fifth of the global atomics.  Samples outside the
+// box (waves that do not march with the slabs, boxes too large for LDS) go
+// straight to the global sums.
+constexpr int kRayBpCap = 2048;  // voxels of an LDS slab box: two 64-bit sums each (32 KB)
+
+// SingleScalarVolume's eight trilinear weights of a clamped position, the
+// products in tri_corners' order.
+__device__ __forceinline__ void tri_weights(float cx, float cy, float cz, float w[8]) {
+    const float t0 = __builtin_amdgcn_fractf(cx), t1 = __builtin_amdgcn_fractf(cy), t2 = __builtin_amdgcn_fractf(cz);
+    const float m0 = 1.0f - t0, m1 = 1.0f - t1, m2 = 1.0f - t2;
+    w[0] = m0 * m1 * m2;
+    w[1] = t0 * m1 * m2;
+    w[2] = m0 * t1 * m2;
+    w[3] = t0 * t1 * m2;
+    w[4] = m0 * m1 * t2;
+    w[5] = t0 * m1 * t2;
+    w[6] = m0 * t1 * t2;
+    w[7] = t0 * t1 * t2;
+}
+
+// One sample's eight taps into the LDS box at index li (cell (x,y,z) - lo).
+__device__ __forceinline__ void lds_scatter(unsigned long long* lc, unsigned long long* lw, int li, int px, int pxy,
+                                            const float w[8], float val, double cscale, double wscale) {
+    const int off[8] = {0, 1, px, px + 1, pxy, pxy + 1, pxy + px, pxy + px + 1};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        __hip_atomic_fetch_add(lc + li + off[n], fixed_of(w[n] * val, cscale), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(lw + li + off[n], fixed_of(w[n], wscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+template <int D, bool DOWN, int NT, int CAP>
+__device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, bool hit, const SinkFixed& sink,
+                                             unsigned long long* lc, unsigned long long* lw, float* red, int* slots) {
+    constexpr int A = D == 0 ? 1 : 0;
+    constexpr int B = D == 2 ? 1 : 2;
+    const float bnd[3] = {g.bx, g.by, g.bz};
+    RayGen G;
+    bool entry_alone;
+    bool have = raygen_init(G, r, g, hit, entry_alone);
+    int n = 0;
+    if (entry_alone) {  // pre-entry case: the entry sample first, on its own
+        float e0, e1, e2;
+        {
+#pragma clang fp contract(off)
+            e0 = r.o[0] + r.d[0] * r.tnear;
+            e1 = r.o[1] + r.d[1] * r.tnear;
+            e2 = r.o[2] + r.d[2] * r.tnear;
+        }
+        sink(g, e0, e1, e2);
+        n = 1;
+    } else if (have) {
+        // the entry sample coincides with the pending one and is scattered
+        // twice (CastingRay's entry sample, then the grid's first); the first
+        // copy here, so every sample of the loops below has weight 1
+        sink(g, G.pend[0], G.pend[1], G.pend[2]);
+        n = 1;
+        G.w = 1.0f;
+    }
+    // slopes of the rays that march with the slabs and the first base plane
+    // (as slab_ray)
+    const bool fwd = DOWN ? r.d[D] < 0.0f : r.d[D] > 0.0f;
+    float q[6];
+    const bool qmax[6] = {false, true, false, true, DOWN, DOWN};
+    {
+        const float sa = r.d[A] / r.d[D], sb = r.d[B] / r.d[D];
+        const bool use = have && fwd;
+        q[0] = use ? sa : INFINITY;
+        q[1] = use ? sa : -INFINITY;
+        q[2] = use ? sb : INFINITY;
+        q[3] = use ? sb : -INFINITY;
+        const float pD = floorf(__builtin_amdgcn_fmed3f(G.pend[D], 0.0f, bnd[D]));
+        q[4] = have ? pD : (DOWN ? -INFINITY : INFINITY);
+        q[5] = use ? G.pend[D] : (DOWN ? -INFINITY : INFINITY);
+    }
+    block_minmax<NT / 64, 6>(q, qmax, red);
+    float o[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r.o[c])));
+    const bool slopes_ok = q[0] <= q[1] && q[2] <= q[3];
+    const bool any_ray = DOWN ? q[4] > -INFINITY : q[4] < INFINITY;
+    int c0 = (int)fminf(fmaxf(q[4], -1.0f), (float)g.nx + g.ny + g.nz);
+    const int dimD = D == 0 ? g.nx : (D == 1 ? g.ny : g.nz);
+    const float sdw = g.sd;
+    const float cmax = 2.0f * (fmaxf(g.bx, fmaxf(g.by, g.bz)) + 1.0f);
+    const bool near = fmaxf(fabsf(G.pend[0]), fmaxf(fabsf(G.pend[1]), fabsf(G.pend[2]))) <= cmax;
+    const bool wave_in_box = kSlabUncheckedPath && __builtin_amdgcn_ballot_w64(have && !(fwd && near)) == 0;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
+    if (any_ray) {
+        if (wv == 0) {
+            SlabBox b0 = slab_box<D, DOWN, CAP>(g, c0, o, q[0], q[1], q[2], q[3], q[5]);
+            if (!slopes_ok) b0.staged = false;
+            box_put(slots, b0);
+        }
+        __syncthreads();
+    }
+#pragma unroll 1
+    for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
+        const SlabBox bx = box_get<DOWN>(slots + kBoxInts * (guard & 1));
+        if (wv == (guard & (NT / 64 - 1))) {
+            SlabBox nb = slab_box<D, DOWN, CAP>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);
+            if (!slopes_ok) nb.staged = false;
+            box_put(slots + kBoxInts * ((guard + 1) & 1), nb);
+        }
+        __syncthreads();  // the box is published; the previous slab's flush has cleared the LDS sums
+        const int base = bx.base;
+        if (bx.staged && wave_in_box) {
+            // every sample of the slab lies in the box (as slab_walk)
+            const float lim = DOWN ? (float)(bx.cend + 1) : (float)bx.cend;
+            constexpr float kEnded = DOWN ? INFINITY : -INFINITY;
+            float llim = have ? lim : kEnded;
+#pragma unroll 1
+            for (;;) {
+                const float cx = __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx);
+                const float cy = __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by);
+                const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
+                const float cc[3] = {cx, cy, cz};
+                if (!(DOWN ? cc[D] >= llim : cc[D] < llim)) break;
+                const int li = (int)cx + __mul24((int)cy, bx.px) + __mul24((int)cz, bx.pxy) - base;
+                float w[8];
+                tri_weights(cx, cy, cz, w);
+                lds_scatter(lc, lw, li, bx.px, bx.pxy, w, sink.val, sink.cscale, sink.wscale);
+                G.t += sdw;
+                G.pend[0] += G.step[0];
+                G.pend[1] += G.step[1];
+                G.pend[2] += G.step[2];
+                if (__builtin_expect(!(G.t < G.tb), 0)) {
+                    if (!raygen_next_segment(G, sdw)) llim = kEnded;
+                }
+            }
+            have = llim != kEnded;
+        } else {
+            // one wave-uniform loop per slab: each iteration every lane whose
+            // pending sample lies in the slab (or behind it) takes it (as
+            // slab_ray's checked loop)
+#pragma unroll 1
+            for (int itn = 0; itn <= g.max_samples; ++itn) {
+                const float cx = __builtin_amdgcn_fmed3f(G.pend[0], 0.0f, g.bx);
+                const float cy = __builtin_amdgcn_fmed3f(G.pend[1], 0.0f, g.by);
+                const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
+                const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+                const int ic[3] = {ix, iy, iz};
+                const int p = ic[D];
+                const bool take = have && (DOWN ? p > bx.cend : p < bx.cend);
+                if (__builtin_amdgcn_ballot_w64(take) == 0) break;
+                if (take) {
+                    const bool in_slab = DOWN ? p <= bx.cbeg : p >= bx.cbeg;
+                    const bool in_box = bx.staged && in_slab && (unsigned)(ic[A] - bx.lo[A]) < (unsigned)(bx.n[A] - 1) &&
+                                        (unsigned)(ic[B] - bx.lo[B]) < (unsigned)(bx.n[B] - 1);
+                    if (in_box) {
+                        float w[8];
+                        tri_weights(cx, cy, cz, w);
+                        lds_scatter(lc, lw, ix + __mul24(iy, bx.px) + __mul24(iz, bx.pxy) - base, bx.px, bx.pxy, w,
+                                    sink.val, sink.cscale, sink.wscale);
+                    } else {
+                        sink(g, G.pend[0], G.pend[1], G.pend[2]);
+                    }
+                    ++n;
+                    G.w = 1.0f;
+                    G.t += sdw;
+                    G.pend[0] += G.step[0];
+                    G.pend[1] += G.step[1];
+                    G.pend[2] += G.step[2];
+                    if (__builtin_expect(!(G.t < G.tb), 0)) have = raygen_next_segment(G, sdw);
+                    have = have && n < g.max_samples;
+                }
+            }
+        }
+        const bool more = __syncthreads_or(have);  // every LDS addend of the slab is in
+        if (bx.staged) {
+            // flush: rows (y, z) of the box to the waves, x to the lanes; each
+            // non-zero voxel once into the global sums, then cleared
+            const int n0 = bx.n[0], n1 = bx.n[1], rows = n1 * bx.n[2];
+            gu64* ac = (gu64*)sink.acc_c;
+            gu64* aw = (gu64*)sink.acc_w;
+#pragma unroll 1
+            for (int row = wv; row < rows; row += NT / 64) {
+                const int jy = row % n1, kz = row / n1;  // wave-uniform
+                const int e0 = bx.px * jy + bx.pxy * kz;
+                const unsigned q0 = (unsigned)bx.lo[0] + nx * (unsigned)(bx.lo[1] + jy) + nxy * (unsigned)(bx.lo[2] + kz);
+                for (int ix = lane; ix < n0; ix += 64) {
+                    const unsigned long long cv = lc[e0 + ix], wvv = lw[e0 + ix];
+                    if ((cv | wvv) != 0ull) {
+                        __hip_atomic_fetch_add(ac + q0 + ix, cv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(aw + q0 + ix, wvv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        lc[e0 + ix] = 0ull;
+                        lw[e0 + ix] = 0ull;
+                    }
+                }
+            }
+        }
+        c0 = bx.cend;
+        if (!more) break;
+    }
+}
+
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void k_raybp_slab(const WorkItem* __restrict__ items, VolGeom g) {
+    constexpr int NT = TW * TH;
+    __shared__ unsigned long long acc[2 * kRayBpCap];
+    __shared__ float red[6 * NT / 64];
+    __shared__ int slots[2 * kBoxInts];
+    for (int e = threadIdx.x; e < 2 * kRayBpCap; e += NT) acc[e] = 0ull;  // published by the first barrier below
+    int i, j;
+    int item;
+    const bool valid = tile_pixel<TW, TH>(g, i, j, item);
+    const WorkItem& it = items[item];
+    const ViewRec& v = *it.view;
+    Ray r;
+    produce_ray(v, g, (float)(valid ? i : 0), (float)(valid ? j : 0), r);
+    const bool hit = valid && r.tnear < r.tfar;
+    const SinkFixed sink{it.acc_c, it.acc_w, valid ? gptr(it.corr)[i + j * g.W] : 0.0f, ldexp(1.0, *it.acc_shift),
+                         ldexp(1.0, g.acc_wshift)};
+    unsigned long long* lc = acc;
+    unsigned long long* lw = acc + kRayBpCap;
+    switch (v.march & 7) {
+        case 0: slab_scatter<0, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 4: slab_scatter<0, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 1: slab_scatter<1, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 5: slab_scatter<1, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 6: slab_scatter<2, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        default: slab_scatter<2, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+    }
+}
+
 // x <-> z transpose of the volumes of the x-dominant views in a work table
 // (index x + nx*(y + ny*z) -> z + nz*(y + ny*x)) through a 32x33 LDS tile.
 __global__ __launch_bounds__(256) void k_transpose_xz(const WorkItem* __restrict__ items, VolGeom g) {
@@ -2520,7 +2758,12 @@ hipError_t launch_fp(int mode, const WorkItem* items, int nitems, const VolGeom&
             return g.fp_tile <= 0 ? launch_fp_slab<FP_MODE_SART>(items, nitems, g, factor, s)
                                   : launch_fp_level<FP_MODE_SART>(items, nitems, g, factor, s);
         case FP_MODE_SCATTER: hipLaunchKernelGGL((k_fp<FP_MODE_SCATTER, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
-        case FP_MODE_RAYBP: hipLaunchKernelGGL((k_fp<FP_MODE_RAYBP, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor); break;
+        case FP_MODE_RAYBP:
+            // slab shapes: LDS-accumulated slab scatter; level shapes: one
+            // global atomic per sample and tap (same sums bit for bit)
+            if (g.fp_tile <= 0) hipLaunchKernelGGL((k_raybp_slab<TX, TY>), grid, dim3(TX * TY), 0, s, items, g);
+            else hipLaunchKernelGGL((k_fp<FP_MODE_RAYBP, TX, TY>), grid, dim3(TX * TY), 0, s, items, g, factor);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
