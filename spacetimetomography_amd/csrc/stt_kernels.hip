@@ -786,25 +786,6 @@ __device__ __forceinline__ void block_minmax(float v[NV], const bool is_max[NV],
     }
 }
 
-// Block-wide OR of p with one barrier: each wave's lane 0 publishes whether
-// any lane of the wave holds p in the wave's slot of flags[parity], and after
-// the barrier every wave ORs the NW slots.  (The library's __syncthreads_or
-// takes three barriers, LDS atomics and two loads of the workgroup size from
-// the dispatch packet.)  Callers alternate parity, so flags[parity] is
-// rewritten two calls later at the earliest: the rewriting wave has passed
-// the barrier of the call in between, which every wave reaches only after
-// its reads of flags[parity].
-template <int NW>
-__device__ __forceinline__ bool block_any(bool p, int* flags, int parity) {
-    const bool wave_any = __builtin_amdgcn_ballot_w64(p) != 0;
-    if ((threadIdx.x & 63) == 0) flags[parity * NW + (threadIdx.x >> 6)] = wave_any ? 1 : 0;
-    __syncthreads();
-    int a = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) a |= flags[parity * NW + w];
-    return __builtin_amdgcn_readfirstlane(a) != 0;
-}
-
 // Box of the slab whose sample base planes start at c0 (uniform arithmetic);
 // CAP floats of LDS hold it.
 template <int D, bool DOWN, int CAP>
@@ -1003,7 +984,7 @@ __device__ __forceinline__ void slab_walk(RayGen& G, const VolGeom& g, const Sla
 // the line integral, so its unchecked loop keeps no count.
 template <int D, bool DOWN, int NT, bool COUNT, int CAP>
 __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, const Ray& r, bool hit, int& nsamp,
-                                          float* box, float* red, int* slots, int* anyf) {
+                                          float* box, float* red, int* slots) {
     constexpr int A = D == 0 ? 1 : 0;
     constexpr int B = D == 2 ? 1 : 2;
     const VolView vv = vol_view(it.vol, g);
@@ -1113,7 +1094,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
             else slab_walk<D, DOWN, COUNT, true>(G, g, bx, box, lim, nlim, llim, nf, density);
             have = COUNT ? nf < nlim : llim != kEnded;
             c0 = bx.cend;
-            if (!block_any<NT / 64>(have, anyf, guard & 1)) break;
+            if (!__syncthreads_or(have)) break;
             continue;
         }
         n = (int)nf;
@@ -1166,7 +1147,7 @@ __device__ __forceinline__ float slab_ray(const WorkItem& it, const VolGeom& g, 
         }
         nf = (float)n;
         c0 = bx.cend;
-        if (!block_any<NT / 64>(have, anyf, guard & 1)) break;
+        if (!__syncthreads_or(have)) break;
     }
     nsamp = (int)nf;
     return density;
@@ -1178,7 +1159,6 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     __shared__ float box[CAP];
     __shared__ float red[6 * NT / 64];
     __shared__ int slots[2 * kBoxInts];
-    __shared__ int anyf[2 * NT / 64];
     int i, j;
     int item;
     const bool valid = tile_pixel<TW, TH>(g, i, j, item);
@@ -1191,12 +1171,12 @@ __global__ __launch_bounds__(TW* TH) void k_fp_slab(const WorkItem* __restrict__
     int ns = 0;
     float value;
     switch (march) {
-        case 0: value = slab_ray<0, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots, anyf); break;
-        case 4: value = slab_ray<0, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots, anyf); break;
-        case 1: value = slab_ray<1, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots, anyf); break;
-        case 5: value = slab_ray<1, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots, anyf); break;
-        case 6: value = slab_ray<2, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots, anyf); break;
-        default: value = slab_ray<2, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots, anyf); break;
+        case 0: value = slab_ray<0, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 4: value = slab_ray<0, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 1: value = slab_ray<1, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 5: value = slab_ray<1, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        case 6: value = slab_ray<2, true, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
+        default: value = slab_ray<2, false, NT, MODE == FP_MODE_FP, CAP>(it, g, r, hit, ns, box, red, slots); break;
     }
     if (!valid) return;
     const int k = i + j * g.W;
@@ -1265,8 +1245,7 @@ __device__ __forceinline__ void lds_scatter(unsigned long long* lc, unsigned lon
 
 template <int D, bool DOWN, int NT, int CAP>
 __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, bool hit, const SinkFixed& sink,
-                                             unsigned long long* lc, unsigned long long* lw, float* red, int* slots,
-                                             int* anyf) {
+                                             unsigned long long* lc, unsigned long long* lw, float* red, int* slots) {
     constexpr int A = D == 0 ? 1 : 0;
     constexpr int B = D == 2 ? 1 : 2;
     const float bnd[3] = {g.bx, g.by, g.bz};
@@ -1403,7 +1382,7 @@ __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, boo
                 }
             }
         }
-        const bool more = block_any<NT / 64>(have, anyf, guard & 1);  // every LDS addend of the slab is in
+        const bool more = __syncthreads_or(have);  // every LDS addend of the slab is in
         if (bx.staged) {
             // flush: rows (y, z) of the box to the waves, x to the lanes; each
             // non-zero voxel once into the global sums, then cleared
@@ -1437,7 +1416,6 @@ __global__ __launch_bounds__(TW* TH) void k_raybp_slab(const WorkItem* __restric
     __shared__ unsigned long long acc[2 * kRayBpCap];
     __shared__ float red[6 * NT / 64];
     __shared__ int slots[2 * kBoxInts];
-    __shared__ int anyf[2 * NT / 64];
     for (int e = threadIdx.x; e < 2 * kRayBpCap; e += NT) acc[e] = 0ull;  // published by the first barrier below
     int i, j;
     int item;
@@ -1452,12 +1430,12 @@ __global__ __launch_bounds__(TW* TH) void k_raybp_slab(const WorkItem* __restric
     unsigned long long* lc = acc;
     unsigned long long* lw = acc + kRayBpCap;
     switch (v.march & 7) {
-        case 0: slab_scatter<0, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots, anyf); break;
-        case 4: slab_scatter<0, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots, anyf); break;
-        case 1: slab_scatter<1, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots, anyf); break;
-        case 5: slab_scatter<1, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots, anyf); break;
-        case 6: slab_scatter<2, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots, anyf); break;
-        default: slab_scatter<2, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots, anyf); break;
+        case 0: slab_scatter<0, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 4: slab_scatter<0, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 1: slab_scatter<1, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 5: slab_scatter<1, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        case 6: slab_scatter<2, true, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
+        default: slab_scatter<2, false, NT, kRayBpCap>(g, r, hit, sink, lc, lw, red, slots); break;
     }
 }
 
