@@ -10,12 +10,12 @@
 // cubicInterpolate, double m_halfFrustumWidth/m_ds/m_alpha); R = double is
 // the FP64 oracle (all arithmetic in double).
 //
-// Parity unpinned: the optical-flow solver's stopping residual
-// err = (d^2 + p^2) / nPx of its primal and dual residual sums
-// (optical_flow.h:359-362, 465-486, 500-502), which sets the iteration counts,
-// is restated without an independent check; the solver's fixed point, its
-// warp, gradient, blur and resize are pinned (closed forms, a duality-gap
-// certificate, and the reference's own code compiled in oracle/_ref).
+// Pins of the optical-flow solver: its warp, gradient, blur and resize by
+// closed forms and the reference's own code compiled in oracle/_ref; its fixed
+// point by a duality-gap certificate; its stopping residual
+// err = (d^2 + p^2) / nPx (optical_flow.h:359-362, 465-486, 500-502) by an
+// FP64 recomputation of the adaptive-PDHG primal-dual residual from three
+// consecutive iterates (tests/test_of_oracle.py).
 #include "stt_oracle.h"
 
 #include <algorithm>

@@ -248,3 +248,87 @@ def test_of_level_fixed_point_has_a_zero_duality_gap(norm):
     assert certificate(v + 0.02 * rng.standard_normal(v.shape) * box, y)[0] > 1e-2
     if norm == 4:
         assert certificate(*solve(0.5))[0] > 1e-2
+
+
+@needs_ref
+@pytest.mark.parametrize("norm,lam", [(4, 0.05), (1, 0.2)])
+def test_of_level_stopping_residual_is_the_primal_dual_residual(norm, lam):
+    """The solver's convergence test (optical_flow.h:359-362, 465-486,
+    500-502) is the primal-dual residual of adaptive PDHG (Goldstein, Li,
+    Yuan, Esser, Baraniuk, "Adaptive primal-dual splitting methods for
+    statistical learning and image processing", 2015) with the loop's
+    diagonal steps: for the iterates v^k, y^k after iteration k, at every
+    50th iteration
+        P = (v^{k-1} - v^k) / tau - K^T (y^{k-2} - y^{k-1})   (the primal step of
+            iteration k reads y^{k-1}; on the ROI),
+        D = (y^{k-1} - y^k) / sigma - K (v^{k-1} - v^k)   (every voxel; sigma1 on
+            the nine TV rows, sigma2 on the data row),
+        err = (|D|_1^2 + |P|_1^2) / n.
+    Recomputed here in FP64 from the oracle's iterates after 48, 49 and 50
+    iterations, with K = (forward differences of each flow component, grad(I2w)
+    . v) assembled from the differences pinned against the reference's
+    tools.cpp and CImg's central gradient; tau = 1/(4 + |grad_c|), sigma1 =
+    1/3, sigma2 = 1/sum|grad| (optical_flow.h:169, 285-291).  Readings that
+    differ in one term -- the data row dropped from D, sigma2 on the TV rows,
+    P without its K^T term, squares instead of absolute values -- move err by
+    far more than the tolerance, which the test checks too."""
+    dims = (10, 9, 8)
+    n = int(np.prod(dims))
+    ext = np.asarray(dims, np.float32)
+    I1, I2 = _smooth_pair(dims, (0.5, -0.3, 0.2))
+    V0 = np.zeros(3 * n, np.float32)
+    Y0 = np.zeros(10 * n, np.float32)
+    roi = (0,) * 6  # the write-back covers every voxel, so the returned iterates are the loop's
+
+    def run(max_iter):
+        V, Y, its, errs = O.of_level(dims, ext, I1, I2, V0, Y0, tol=0.0, lam=lam, max_iter=max_iter, norm=norm,
+                                     warps=1, huber=0.1, roi=roi)
+        assert its[0] == max_iter + 1  # while (err > tol && iterations <= maxIterations) ++iterations
+        return V.astype(np.float64).reshape(3, n), Y.astype(np.float64).reshape(10, n), float(errs[0])
+
+    _, y48, _ = run(47)
+    v49, y49, _ = run(48)
+    v50, y50, err = run(49)
+    W = O.warp_ext(dims, ext, V0, I2, -1.0)
+    u = np.stack([g.astype(np.float64) for g in O.gradient_central(dims, W)])
+    tau = 1.0 / (4.0 + np.abs(u))
+    # sigma2 is infinite where the warped image is flat (the volume's corners,
+    # whose clamped warp samples equal their neighbours'): the data row's
+    # (y4 change) / sigma2 is 0 there, as in the loop
+    with np.errstate(divide="ignore"):
+        sigma1, sigma2 = 1.0 / 3.0, 1.0 / np.abs(u).sum(0)
+
+    def diff(a, which):
+        return O.of_diff(dims, np.ascontiguousarray(a, np.float32), which).astype(np.float64)
+
+    def K(v):  # 10 rows: D_x, D_y, D_z of each component, then grad(I2w) . v
+        rows = [diff(v[c], ax) for c in range(3) for ax in range(3)]
+        return np.stack(rows + [(u * v).sum(0)])
+
+    def KT(y):  # -div of each component's TV dual + grad_c * y4
+        return np.stack([-(diff(y[3 * c], 3) + diff(y[3 * c + 1], 4) + diff(y[3 * c + 2], 5)) + u[c] * y[9]
+                         for c in range(3)])
+
+    P = (v49 - v50) / tau - (KT(y48) - KT(y49))
+    Dtv = (y49[:9] - y50[:9]) / sigma1 - (K(v49)[:9] - K(v50)[:9])
+    Ddat = (y49[9] - y50[9]) / sigma2 - (K(v49)[9] - K(v50)[9])
+
+    def residual(p, d):
+        return (d * d + p * p) / n
+
+    p1, d1 = np.abs(P).sum(), np.abs(Dtv).sum() + np.abs(Ddat).sum()
+    ref = residual(p1, d1)
+    rel = abs(err - ref) / ref
+    print(f"norm {norm}: oracle err {err:.8e}, recomputed {ref:.8e}, relative difference {rel:.1e} "
+          f"(|P|_1 {p1:.4e}, |D|_1 {d1:.4e})")
+    tol = 1e-5  # fp32 iterates and per-voxel terms against their FP64 recomputation
+    assert rel <= tol, (err, ref)
+    variants = {
+        "data row dropped": residual(p1, np.abs(Dtv).sum()),
+        "sigma2 on the TV rows": residual(
+            p1, np.abs((y49[:9] - y50[:9]) / sigma2 - (K(v49)[:9] - K(v50)[:9])).sum() + np.abs(Ddat).sum()),
+        "P without K^T": residual(np.abs((v49 - v50) / tau).sum(), d1),
+        "squares": residual((P * P).sum(), (Dtv * Dtv).sum() + (Ddat * Ddat).sum()),
+    }
+    for name, val in variants.items():
+        assert abs(val - ref) > 100 * tol * ref, (name, val, ref)
