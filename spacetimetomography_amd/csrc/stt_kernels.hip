@@ -788,7 +788,10 @@ __device__ __forceinline__ void block_minmax(float v[NV], const bool is_max[NV],
 
 // Box of the slab whose sample base planes start at c0 (uniform arithmetic);
 // CAP floats of LDS hold it.
-template <int D, bool DOWN, int CAP>
+// PADDED: the projector's x-fastest LDS layout (odd row and plane pitches);
+// false: a dense layout with the slab axis slowest (the ray-BP scatter box),
+// whose footprint is exactly n[A] * n[B] * n[D].
+template <int D, bool DOWN, int CAP, bool PADDED = true>
 __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const float o[3], float sAmin, float sAmax,
                                             float sBmin, float sBmax, float dfirst) {
     constexpr int A = D == 0 ? 1 : 0;
@@ -835,7 +838,7 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     if (nA > 0 && nB > 0) {
         // thinner slab when the box would not fit; its true box only shrinks,
         // so the extents above stay a valid (conservative) bound
-        const int per_plane = (nA | 1) * nB;
+        const int per_plane = PADDED ? (nA | 1) * nB : nA * nB;
         if (per_plane * (K + 1) > CAP) {
             K = (int)((float)CAP * __builtin_amdgcn_rcpf((float)per_plane)) - 1;
             while (K > 0 && per_plane * (K + 1) > CAP) --K;
@@ -863,8 +866,12 @@ __device__ __forceinline__ SlabBox slab_box(const VolGeom& g, int c0, const floa
     bx.n[B] = nB;
     bx.px = bx.n[0] | 1;
     bx.pxy = (bx.px * bx.n[1]) | 1;
-    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= CAP &&
-                bx.px * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
+    if (PADDED)
+        bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && bx.pxy * bx.n[2] <= CAP &&
+                    bx.px * bx.n[1] <= 256 * kStagePasses;  // the smallest block stages 256 per pass
+    else
+        bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && nA <= CAP && nB <= CAP &&
+                    nA * nB * bx.n[D] <= CAP;  // each factor bounded first: no int overflow
     bx.base = bx.lo[0] + bx.px * bx.lo[1] + bx.pxy * bx.lo[2];
     // Inner slab: the sampler's clamp to [0, dims-1] is the identity on all
     // of its samples.  Transversally the samples lie within eps of the
@@ -1231,10 +1238,11 @@ __device__ __forceinline__ void tri_weights(float cx, float cy, float cz, float 
     w[7] = t0 * t1 * t2;
 }
 
-// One sample's eight taps into the LDS box at index li (cell (x,y,z) - lo).
-__device__ __forceinline__ void lds_scatter(unsigned long long* lc, unsigned long long* lw, int li, int px, int pxy,
-                                            const float w[8], float val, double cscale, double wscale) {
-    const int off[8] = {0, 1, px, px + 1, pxy, pxy + 1, pxy + px, pxy + px + 1};
+// One sample's eight taps into the LDS box at index li (cell (x,y,z) - lo);
+// sx, sy, sz: the box's strides along the volume axes.
+__device__ __forceinline__ void lds_scatter(unsigned long long* lc, unsigned long long* lw, int li, int sx, int sy,
+                                            int sz, const float w[8], float val, double cscale, double wscale) {
+    const int off[8] = {0, sx, sy, sy + sx, sz, sz + sx, sz + sy, sz + sy + sx};
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         __hip_atomic_fetch_add(lc + li + off[n], fixed_of(w[n] * val, cscale), __ATOMIC_RELAXED,
@@ -1304,7 +1312,7 @@ __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, boo
     const unsigned nx = (unsigned)g.nx, nxy = (unsigned)(g.nx * g.ny);
     if (any_ray) {
         if (wv == 0) {
-            SlabBox b0 = slab_box<D, DOWN, CAP>(g, c0, o, q[0], q[1], q[2], q[3], q[5]);
+            SlabBox b0 = slab_box<D, DOWN, CAP, false>(g, c0, o, q[0], q[1], q[2], q[3], q[5]);
             if (!slopes_ok) b0.staged = false;
             box_put(slots, b0);
         }
@@ -1314,12 +1322,19 @@ __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, boo
     for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
         const SlabBox bx = box_get<DOWN>(slots + kBoxInts * (guard & 1));
         if (wv == (guard & (NT / 64 - 1))) {
-            SlabBox nb = slab_box<D, DOWN, CAP>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);
+            SlabBox nb = slab_box<D, DOWN, CAP, false>(g, bx.cend, o, q[0], q[1], q[2], q[3], DOWN ? -INFINITY : INFINITY);
             if (!slopes_ok) nb.staged = false;
             box_put(slots + kBoxInts * ((guard + 1) & 1), nb);
         }
         __syncthreads();  // the box is published; the previous slab's flush has cleared the LDS sums
-        const int base = bx.base;
+        // dense box layout, the slab axis D slowest: index of cell c is
+        // ((c[D] - lo[D]) * n[B] + c[B] - lo[B]) * n[A] + c[A] - lo[A]
+        int st[3];
+        st[A] = 1;
+        st[B] = bx.n[A];
+        st[D] = bx.n[A] * bx.n[B];
+        const int sx = st[0], sy = st[1], sz = st[2];
+        const int base = bx.lo[0] * sx + bx.lo[1] * sy + bx.lo[2] * sz;
         if (bx.staged && wave_in_box) {
             // every sample of the slab lies in the box (as slab_walk)
             const float lim = DOWN ? (float)(bx.cend + 1) : (float)bx.cend;
@@ -1332,10 +1347,10 @@ __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, boo
                 const float cz = __builtin_amdgcn_fmed3f(G.pend[2], 0.0f, g.bz);
                 const float cc[3] = {cx, cy, cz};
                 if (!(DOWN ? cc[D] >= llim : cc[D] < llim)) break;
-                const int li = (int)cx + __mul24((int)cy, bx.px) + __mul24((int)cz, bx.pxy) - base;
+                const int li = __mul24((int)cx, sx) + __mul24((int)cy, sy) + __mul24((int)cz, sz) - base;
                 float w[8];
                 tri_weights(cx, cy, cz, w);
-                lds_scatter(lc, lw, li, bx.px, bx.pxy, w, sink.val, sink.cscale, sink.wscale);
+                lds_scatter(lc, lw, li, sx, sy, sz, w, sink.val, sink.cscale, sink.wscale);
                 G.t += sdw;
                 G.pend[0] += G.step[0];
                 G.pend[1] += G.step[1];
@@ -1366,7 +1381,7 @@ __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, boo
                     if (in_box) {
                         float w[8];
                         tri_weights(cx, cy, cz, w);
-                        lds_scatter(lc, lw, ix + __mul24(iy, bx.px) + __mul24(iz, bx.pxy) - base, bx.px, bx.pxy, w,
+                        lds_scatter(lc, lw, __mul24(ix, sx) + __mul24(iy, sy) + __mul24(iz, sz) - base, sx, sy, sz, w,
                                     sink.val, sink.cscale, sink.wscale);
                     } else {
                         sink(g, G.pend[0], G.pend[1], G.pend[2]);
@@ -1384,23 +1399,27 @@ __device__ __forceinline__ void slab_scatter(const VolGeom& g, const Ray& r, boo
         }
         const bool more = __syncthreads_or(have);  // every LDS addend of the slab is in
         if (bx.staged) {
-            // flush: rows (y, z) of the box to the waves, x to the lanes; each
-            // non-zero voxel once into the global sums, then cleared
-            const int n0 = bx.n[0], n1 = bx.n[1], rows = n1 * bx.n[2];
+            // flush: rows (B, D) of the box to the waves, A to the lanes; each
+            // non-zero voxel once into the global sums, then cleared (taps past
+            // the volume's last cell carry weight 0 and stay zero)
+            const int nA = bx.n[A], nB = bx.n[B], rows = nB * bx.n[D];
+            const unsigned gst[3] = {1u, nx, nxy};  // global strides per volume axis
             gu64* ac = (gu64*)sink.acc_c;
             gu64* aw = (gu64*)sink.acc_w;
 #pragma unroll 1
             for (int row = wv; row < rows; row += NT / 64) {
-                const int jy = row % n1, kz = row / n1;  // wave-uniform
-                const int e0 = bx.px * jy + bx.pxy * kz;
-                const unsigned q0 = (unsigned)bx.lo[0] + nx * (unsigned)(bx.lo[1] + jy) + nxy * (unsigned)(bx.lo[2] + kz);
-                for (int ix = lane; ix < n0; ix += 64) {
-                    const unsigned long long cv = lc[e0 + ix], wvv = lw[e0 + ix];
+                const int jb = row % nB, jd = row / nB;  // wave-uniform
+                const int e0 = row * nA;
+                const unsigned q0 = gst[A] * (unsigned)bx.lo[A] + gst[B] * (unsigned)(bx.lo[B] + jb) +
+                                    gst[D] * (unsigned)(bx.lo[D] + jd);
+                for (int ia = lane; ia < nA; ia += 64) {
+                    const unsigned long long cv = lc[e0 + ia], wvv = lw[e0 + ia];
                     if ((cv | wvv) != 0ull) {
-                        __hip_atomic_fetch_add(ac + q0 + ix, cv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        __hip_atomic_fetch_add(aw + q0 + ix, wvv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        lc[e0 + ix] = 0ull;
-                        lw[e0 + ix] = 0ull;
+                        const unsigned q = q0 + gst[A] * (unsigned)ia;
+                        __hip_atomic_fetch_add(ac + q, cv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(aw + q, wvv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        lc[e0 + ia] = 0ull;
+                        lw[e0 + ia] = 0ull;
                     }
                 }
             }
