@@ -15,6 +15,9 @@
 // bound = 1: the extremes clamped to [0, dims-1] before the floor, the
 // range the clamped sample positions actually take.
 //
+// slab_model_ex also models the ray-BP scatter's box (k_raybp_slab): a
+// capacity in voxels and the dense layout with the slab axis slowest.
+//
 // Inner slabs (SlabBox::inner, walked without the clamp to [0, dims-1]): for
 // every sample of such a slab the clamped and unclamped positions must be
 // equal, and the unclamped slab-membership test must take the same samples
@@ -152,11 +155,11 @@ struct Box {
     bool staged, inner;
 };
 
-constexpr int kSlabCap = 6144, kSlabK = 23, kSlabKMin = 8, kStagePasses = 4;
+constexpr int kSlabK = 23, kSlabKMin = 8, kStagePasses = 4;
 constexpr float kSlabShear = 0.7f;
 
 Box slab_box(const Geo& g, int D, bool down, int c0, const float o[3], float sAmin, float sAmax, float sBmin,
-             float sBmax, float dfirst, int bound) {
+             float sBmax, float dfirst, int bound, int cap, bool dense) {
     const int A = D == 0 ? 1 : 0, B = D == 2 ? 1 : 2;
     Box bx;
     const float e0 = (down ? std::fmax((float)(c0 + 1), dfirst) : std::fmin((float)c0, dfirst)) - o[D];
@@ -187,10 +190,10 @@ Box slab_box(const Geo& g, int D, bool down, int c0, const float o[3], float sAm
     }
     const int nA = hiA - loA + 1, nB = hiB - loB + 1;
     if (nA > 0 && nB > 0) {
-        const int per_plane = (nA | 1) * nB;
-        if (per_plane * (K + 1) > kSlabCap) {
-            K = (int)((float)kSlabCap * (1.0f / (float)per_plane)) - 1;
-            while (K > 0 && per_plane * (K + 1) > kSlabCap) --K;
+        const int per_plane = dense ? nA * nB : (nA | 1) * nB;
+        if (per_plane * (K + 1) > cap) {
+            K = (int)((float)cap * (1.0f / (float)per_plane)) - 1;
+            while (K > 0 && per_plane * (K + 1) > cap) --K;
             K = std::max(K, 1);
         }
     }
@@ -213,7 +216,10 @@ Box slab_box(const Geo& g, int D, bool down, int c0, const float o[3], float sAm
     bx.lo[B] = loB;
     bx.n[B] = nB;
     const int px = bx.n[0] | 1, pxy = (px * bx.n[1]) | 1;
-    bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && pxy * bx.n[2] <= kSlabCap && bx.n[0] * bx.n[1] <= 256 * kStagePasses;
+    if (dense)  // the ray-BP scatter box: slab axis slowest, no padding
+        bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && nA <= cap && nB <= cap && nA * nB * bx.n[D] <= cap;
+    else
+        bx.staged = nA > 0 && nB > 0 && bx.n[D] > 0 && pxy * bx.n[2] <= cap && bx.n[0] * bx.n[1] <= 256 * kStagePasses;
     const float bD = g.bnd[D];
     bx.inner = bx.staged && a0 - eps >= 0.0f && a1 + eps <= g.bnd[A] && b0 - eps >= 0.0f && b1 + eps <= g.bnd[B] &&
                (down ? (c0 - K >= 0 && (float)(c0 + 1) <= bD) : (c0 >= 1 && (float)(c0 + K) <= bD));
@@ -229,8 +235,11 @@ struct Lane {
 }  // namespace
 
 // tw: rays per tile row (16, 32 or 8; 256 / tw rows), the k_fp_slab tile.
-extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nviews, int bound, int tw,
-                          long long* out) {
+// cap: floats (projector) or voxels (ray-BP scatter) of the LDS box;
+// dense: the scatter's layout (slab axis slowest, footprint n[A] n[B] n[D])
+// instead of the projector's padded x-fastest one.
+extern "C" int slab_model_ex(const stt_geometry* geom, const float* angles, int nviews, int bound, int tw, int cap,
+                             int dense, long long* out) {
     if (tw != 16 && tw != 32 && tw != 8) return 1;
     const int th = 256 / tw;
     Geo g;
@@ -302,7 +311,7 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
             const int dimD = g.dims[D];
             for (int guard = 0; any_ray && guard <= dimD + 1; ++guard) {
                 const float dfirst = guard == 0 ? q6[5] : (down ? -INFINITY : INFINITY);
-                Box bx = slab_box(g, D, down, c0, o, q6[0], q6[1], q6[2], q6[3], dfirst, bound);
+                Box bx = slab_box(g, D, down, c0, o, q6[0], q6[1], q6[2], q6[3], dfirst, bound, cap, dense != 0);
                 if (!slopes_ok) bx.staged = false;
                 bool any_left = false;
                 for (int t = 0; t < 256; ++t) {
@@ -360,4 +369,10 @@ extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nvi
     out[4] = inner_samples;
     out[5] = inner_bad;
     return 0;
+}
+
+// The projector's box: 6144 floats, padded layout.
+extern "C" int slab_model(const stt_geometry* geom, const float* angles, int nviews, int bound, int tw,
+                          long long* out) {
+    return slab_model_ex(geom, angles, nviews, bound, tw, 6144, 0, out);
 }

@@ -32,14 +32,16 @@ def model(tmp_path_factory):
                     os.path.join(HERE, "slab_box_model.cpp"),
                     os.path.join(ROOT, "spacetimetomography_amd", "csrc", "stt_geometry.cpp")], check=True)
     lib = C.CDLL(so)
-    lib.slab_model.restype = C.c_int
-    lib.slab_model.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
+    lib.slab_model_ex.restype = C.c_int
+    lib.slab_model_ex.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(C.c_longlong)]
 
-    def run(vol, det, spacing, ds, sid, sdd, angles, bound, oxyz=(0, 0, 0), tw=16):
+    def run(vol, det, spacing, ds, sid, sdd, angles, bound, oxyz=(0, 0, 0), tw=16, cap=6144, dense=False):
         g = stt.Geometry.make(vol, det, spacing, ds, sid, sdd, oxyz)
         a = np.ascontiguousarray(angles, np.float32)
         out = (C.c_longlong * 6)()
-        assert lib.slab_model(C.byref(g), a.ctypes.data_as(C.POINTER(C.c_float)), len(a), bound, tw, out) == 0
+        assert lib.slab_model_ex(C.byref(g), a.ctypes.data_as(C.POINTER(C.c_float)), len(a), bound, tw, cap,
+                                 int(dense), out) == 0
         return dict(checked=out[0], outside=out[1], waves=out[2], samples=out[3], inner=out[4], inner_bad=out[5])
 
     return run
@@ -82,3 +84,23 @@ def test_iteration_geometry_inner_slabs(model, tw):
     print(new, "inner fraction", new["inner"] / new["samples"])
     assert new["outside"] == 0 and new["inner_bad"] == 0
     assert new["inner"] > 0.5 * new["samples"]
+
+
+def test_ray_bp_scatter_boxes_stage_every_sample(model):
+    """The ray-BP scatter's 2048-voxel LDS box in its dense layout (slab axis
+    slowest, footprint n[A] n[B] n[D]) fits every box the slab depth cap
+    accepts: on the iteration benchmark's geometry (256^3, 512^2 at
+    600/1000 mm) and 12 golden-angle views, every sample of every wave goes
+    through a staged box, with no tap outside it.  The projector's padded
+    x-fastest layout at the same capacity leaves a fifth of the samples in
+    refused boxes (the x-dominant views, whose fastest LDS axis is the slab
+    depth), which is what the dense layout removed."""
+    ang = list(stt.synth_angles(1, 12, offset=0))
+    geo = ((256, 256, 256), (512, 512), (1, 1, 1), 1.0, 600.0, 1000.0)
+    dense = model(*geo, ang, 1, cap=2048, dense=True)
+    padded = model(*geo, ang, 1, cap=2048, dense=False)
+    print(dense, padded)
+    assert dense["outside"] == 0 and padded["outside"] == 0
+    assert dense["samples"] == padded["samples"]
+    assert dense["checked"] == dense["samples"]
+    assert padded["checked"] < 0.9 * padded["samples"]
