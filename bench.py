@@ -692,7 +692,9 @@ def main():
         "value_retrace_every_2_passes": 1e3 / (ms_per_step + tr.trace_ms / 2),
         "kernel_ms_per_step": {
             "fp": fp_ms / ksteps, "bp": bp_ms / ksteps, "pd_dual": dual_ms / ksteps, "pd_primal": primal_ms / ksteps,
-            "layout_copy": aux_ms / ksteps,
+            # the x<->z layout copy of a level projector shape; a slab shape
+            # launches none, and the interval holds only the events' own gaps
+            ("layout_copy" if tile > 0 else "event_gaps"): aux_ms / ksteps,
         },
         "iteration_hbm_frac": iter_bytes / (ms_per_step / 1e3) / HBM_PEAK,
     }
