@@ -152,9 +152,15 @@ int  stt_get_volume(stt_ctx* ctx, int frame, float* vol);
  * every change of p.interp makes the next PD step re-trace it. */
 int  stt_set_flow(stt_ctx* ctx, int pair, const float* uvw);
 int  stt_set_projections(stt_ctx* ctx, int frame, const float* b); /* W*H*views[frame] */
-/* Device-pointer variants (borrowed, copied device-to-device). */
+/* Device-pointer variants (borrowed, copied device-to-device, never freed):
+ * the same data entry points for a caller whose buffers already live on the
+ * device (a torch tensor's data_ptr(), a buffer of its own).  Same sizes,
+ * ranges and errors as the host forms; stt_set_projections_dev derives
+ * G_Max from frame 0 as stt_set_projections does. */
 int  stt_set_volume_dev(stt_ctx* ctx, int frame, const float* vol_dev);
 int  stt_get_volume_dev(stt_ctx* ctx, int frame, float* vol_dev);
+int  stt_set_flow_dev(stt_ctx* ctx, int pair, const float* uvw_dev);
+int  stt_set_projections_dev(stt_ctx* ctx, int frame, const float* b_dev);
 
 /* Dual / auxiliary state readout (X_bar, X_prev, Y_tp, Y_bc: one volume each;
  * Y_tv: four planes, the three spatial components then STV's temporal
@@ -195,6 +201,16 @@ int  stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr,
  * the sampled line integral including the sd factor).  VJP of the projector
  * with respect to the volume. */
 int  stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out);
+/* Device-pointer forms of the three operators above (GenerateImage_MT and the
+ * voxel BP of ST_Tomography.cpp:687-731, the projector's transpose):
+ * the caller's device buffers are read and written in place of host arrays,
+ * with the same semantics (raylen read and written; out accumulated). */
+int  stt_forward_project_dev(stt_ctx* ctx, int frame, int view, float* Ax_dev,
+                             float* raylen_dev, float* weight_dev);
+int  stt_backproject_voxel_dev(stt_ctx* ctx, int frame, int view,
+                               const float* corr_dev, double alpha);
+int  stt_fp_vjp_dev(stt_ctx* ctx, int frame, int view, const float* y_dev,
+                    float* out_dev);
 
 /* One SART sweep of frame `frame` (ST_Tomography::SART, :616-808):
  * views in order[0..n) (n <= views[frame]; NULL = identity order), then
@@ -218,6 +234,12 @@ int  stt_warp_frame(stt_ctx* ctx, int pair, float dt, int interp, float* out);
  * of u at clamp(i, 1, n-2). */
 int  stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt,
                   const float* g, float* gsrc, float* guvw);
+/* Device-pointer forms of stt_warp (warping.h:116-145) and stt_warp_vjp. */
+int  stt_warp_dev(stt_ctx* ctx, const float* uvw_dev, const float* src_dev,
+                  float dt, float* out_dev, int interp);
+int  stt_warp_vjp_dev(stt_ctx* ctx, const float* uvw_dev, const float* src_dev,
+                      float dt, const float* g_dev, float* gsrc_dev,
+                      float* guvw_dev);
 
 /* Initialise X_bar <- x for all frames and zero the duals
  * (ST_Tomography.cpp:494-500 first pass, SetData :339-350). */

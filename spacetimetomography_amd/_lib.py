@@ -106,6 +106,13 @@ _SIGS = {
     "stt_set_projections": (C.c_int, [_P, C.c_int, _F]),
     "stt_set_volume_dev": (C.c_int, [_P, C.c_int, _P]),
     "stt_get_volume_dev": (C.c_int, [_P, C.c_int, _P]),
+    "stt_set_flow_dev": (C.c_int, [_P, C.c_int, _P]),
+    "stt_set_projections_dev": (C.c_int, [_P, C.c_int, _P]),
+    "stt_forward_project_dev": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P]),
+    "stt_backproject_voxel_dev": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_double]),
+    "stt_fp_vjp_dev": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "stt_warp_dev": (C.c_int, [_P, _P, _P, C.c_float, _P, C.c_int]),
+    "stt_warp_vjp_dev": (C.c_int, [_P, _P, _P, C.c_float, _P, _P, _P]),
     "stt_get_buffer": (C.c_int, [_P, C.c_int, C.c_int, _F]),
     "stt_set_buffer": (C.c_int, [_P, C.c_int, C.c_int, _F]),
     "stt_forward_project": (C.c_int, [_P, C.c_int, C.c_int, _F, _F, _F]),
@@ -170,6 +177,19 @@ class SttError(RuntimeError):
 def check(rc):
     if rc != STT_OK:
         raise SttError(rc, lib.stt_last_error().decode(errors="replace"))
+
+
+def dptr(x):
+    """A device address for the "_dev" entry points (None passes NULL): an
+    int or a c_void_p holding memory allocated through the HIP runtime that
+    libstt.so links (ROCm's libamdhip64.so.7).  A runtime bundled elsewhere in
+    the process (PyTorch wheels carry their own) has its own address space
+    bookkeeping, so its pointers do not qualify.  The buffer is borrowed; work
+    that writes it must have finished, because the library uses its own
+    stream."""
+    if x is None:
+        return None
+    return x if isinstance(x, C.c_void_p) else C.c_void_p(int(x))
 
 
 def fptr(a):
@@ -404,6 +424,35 @@ class Context:
 
     def set_projections(self, frame, b):
         check(lib.stt_set_projections(self.h, frame, fptr(np.ascontiguousarray(b, np.float32).reshape(-1))))
+
+    # Device-pointer forms (stt_*_dev): arguments are device addresses (dptr).
+    def set_volume_dev(self, frame, vol_dev):
+        check(lib.stt_set_volume_dev(self.h, frame, dptr(vol_dev)))
+
+    def get_volume_dev(self, frame, out_dev):
+        check(lib.stt_get_volume_dev(self.h, frame, dptr(out_dev)))
+
+    def set_flow_dev(self, pair, uvw_dev):
+        check(lib.stt_set_flow_dev(self.h, pair, dptr(uvw_dev)))
+
+    def set_projections_dev(self, frame, b_dev):
+        check(lib.stt_set_projections_dev(self.h, frame, dptr(b_dev)))
+
+    def forward_project_dev(self, frame, view, ax_dev=None, raylen_dev=None, weight_dev=None):
+        check(lib.stt_forward_project_dev(self.h, frame, view, dptr(ax_dev), dptr(raylen_dev), dptr(weight_dev)))
+
+    def backproject_voxel_dev(self, frame, view, corr_dev, alpha):
+        check(lib.stt_backproject_voxel_dev(self.h, frame, view, dptr(corr_dev), alpha))
+
+    def fp_vjp_dev(self, frame, view, y_dev, out_dev):
+        check(lib.stt_fp_vjp_dev(self.h, frame, view, dptr(y_dev), dptr(out_dev)))
+
+    def warp_dev(self, uvw_dev, src_dev, dt, out_dev, interp=STT_INTERP_CUBIC_CLAMPED):
+        check(lib.stt_warp_dev(self.h, dptr(uvw_dev), dptr(src_dev), dt, dptr(out_dev), interp))
+
+    def warp_vjp_dev(self, uvw_dev, src_dev, dt, g_dev, gsrc_dev=None, guvw_dev=None):
+        check(lib.stt_warp_vjp_dev(self.h, dptr(uvw_dev), dptr(src_dev), dt, dptr(g_dev), dptr(gsrc_dev),
+                                   dptr(guvw_dev)))
 
     def get_buffer(self, which, frame):
         n = 4 * self.V if which == STT_BUF_YTV else (3 * self.V if which == STT_BUF_FLOW else self.V)

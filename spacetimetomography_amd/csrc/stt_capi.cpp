@@ -1227,26 +1227,39 @@ int stt_get_volume_dev(stt_ctx* ctx, int frame, float* vol_dev) {
     return STT_OK;
 }
 
-int stt_set_flow(stt_ctx* ctx, int pair, const float* uvw) {
+namespace {
+
+// The buffer entry points below serve both the host-pointer form and the
+// "_dev" form (device pointers, borrowed): `in` and `out` are the copy kinds
+// of the caller's inputs and outputs.
+int set_flow_impl(stt_ctx* ctx, int pair, const float* uvw, hipMemcpyKind in) {
     CTX_CHECK(ctx);
     if (!uvw) return fail(STT_ERR_INVALID, "null flow");
     if (pair < 0 || pair >= ctx->F) return fail(STT_ERR_INVALID, "flow pair out of range");
-    HIPCHK(hipMemcpyAsync(ctx->flow[pair], uvw, 3 * ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->flow[pair], uvw, 3 * ctx->V * sizeof(float), in, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->flow_changed(pair);
     return STT_OK;
 }
 
-int stt_set_projections(stt_ctx* ctx, int frame, const float* b) {
+int set_projections_impl(stt_ctx* ctx, int frame, const float* b, hipMemcpyKind in) {
     CTX_CHECK(ctx);
     if (!b) return fail(STT_ERR_INVALID, "null projections");
     if (frame < 0 || frame >= ctx->F) return fail(STT_ERR_INVALID, "frame out of range");
     const size_t n = ctx->D * ctx->P[frame];
-    HIPCHK(hipMemcpyAsync(ctx->b[frame], b, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->b[frame], b, n * sizeof(float), in, ctx->stream));
     if (frame == 0 && ctx->gmax_from_data) {
         // G_Max = max(frame-0 projections)/4 (ST_Tomography.cpp:356-357)
+        std::vector<float> staged;
+        const float* hb = b;
+        if (in != hipMemcpyHostToDevice) {  // device input: its values, read back once
+            staged.resize(n);
+            HIPCHK(hipMemcpyAsync(staged.data(), ctx->b[frame], n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            hb = staged.data();
+        }
         float mx = -3.402823466e38f;
-        for (size_t k = 0; k < n; ++k) mx = std::max(mx, b[k]);
+        for (size_t k = 0; k < n; ++k) mx = std::max(mx, hb[k]);
         ctx->gmax = mx / 4.0f;
         ctx->pc.gmax = ctx->gmax;
         ctx->invalidate_graph();
@@ -1255,7 +1268,8 @@ int stt_set_projections(stt_ctx* ctx, int frame, const float* b) {
     return STT_OK;
 }
 
-int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* raylen, float* weight) {
+int forward_project_impl(stt_ctx* ctx, int frame, int view, float* Ax, float* raylen, float* weight, hipMemcpyKind in,
+                         hipMemcpyKind out) {
     CTX_CHECK(ctx);
     if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
         return fail(STT_ERR_INVALID, "frame/view out of range");
@@ -1267,7 +1281,7 @@ int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* ray
     dax = (float*)dax_mem.p;
     drl = dax + D;
     dw = drl + D;
-    if (raylen) HIPCHK(hipMemcpyAsync(drl, raylen, D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (raylen) HIPCHK(hipMemcpyAsync(drl, raylen, D * sizeof(float), in, ctx->stream));
     else HIPCHK(hipMemsetAsync(drl, 0, D * sizeof(float), ctx->stream));
     WorkItem it = ctx->make_item(frame, view, true);
     it.ax = dax;
@@ -1276,20 +1290,20 @@ int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* ray
     int rc = ctx->upload_scratch({it});
     if (rc) return rc;
     HIPCHK(launch_fp(FP_MODE_FP, ctx->items_scratch, 1, ctx->vg, 1.0f, ctx->stream));
-    if (Ax) HIPCHK(hipMemcpyAsync(Ax, dax, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (raylen) HIPCHK(hipMemcpyAsync(raylen, drl, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (weight) HIPCHK(hipMemcpyAsync(weight, dw, D * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (Ax) HIPCHK(hipMemcpyAsync(Ax, dax, D * sizeof(float), out, ctx->stream));
+    if (raylen) HIPCHK(hipMemcpyAsync(raylen, drl, D * sizeof(float), out, ctx->stream));
+    if (weight) HIPCHK(hipMemcpyAsync(weight, dw, D * sizeof(float), out, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
 }
 
-int stt_backproject_voxel(stt_ctx* ctx, int frame, int view, const float* corr, double alpha) {
+int backproject_voxel_impl(stt_ctx* ctx, int frame, int view, const float* corr, double alpha, hipMemcpyKind in) {
     CTX_CHECK(ctx);
     if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
         return fail(STT_ERR_INVALID, "frame/view out of range");
     if (!corr) return fail(STT_ERR_INVALID, "null correction");
     if (!ctx->views_set[frame]) return fail(STT_ERR_STATE, "views of this frame not set");
-    HIPCHK(hipMemcpyAsync(ctx->corr_f[frame], corr, ctx->D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->corr_f[frame], corr, ctx->D * sizeof(float), in, ctx->stream));
     WorkItem it = ctx->make_item(frame, view, true);
     it.last = 0;
     int rc = ctx->upload_scratch({it});
@@ -1299,6 +1313,34 @@ int stt_backproject_voxel(stt_ctx* ctx, int frame, int view, const float* corr, 
     HIPCHK(launch_bp_voxel(ctx->items_scratch, 1, ctx->vg, pc, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
+}
+
+}  // namespace
+
+int stt_set_flow(stt_ctx* ctx, int pair, const float* uvw) {
+    return set_flow_impl(ctx, pair, uvw, hipMemcpyHostToDevice);
+}
+int stt_set_flow_dev(stt_ctx* ctx, int pair, const float* uvw_dev) {
+    return set_flow_impl(ctx, pair, uvw_dev, hipMemcpyDeviceToDevice);
+}
+int stt_set_projections(stt_ctx* ctx, int frame, const float* b) {
+    return set_projections_impl(ctx, frame, b, hipMemcpyHostToDevice);
+}
+int stt_set_projections_dev(stt_ctx* ctx, int frame, const float* b_dev) {
+    return set_projections_impl(ctx, frame, b_dev, hipMemcpyDeviceToDevice);
+}
+int stt_forward_project(stt_ctx* ctx, int frame, int view, float* Ax, float* raylen, float* weight) {
+    return forward_project_impl(ctx, frame, view, Ax, raylen, weight, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+int stt_forward_project_dev(stt_ctx* ctx, int frame, int view, float* Ax_dev, float* raylen_dev, float* weight_dev) {
+    return forward_project_impl(ctx, frame, view, Ax_dev, raylen_dev, weight_dev, hipMemcpyDeviceToDevice,
+                                hipMemcpyDeviceToDevice);
+}
+int stt_backproject_voxel(stt_ctx* ctx, int frame, int view, const float* corr, double alpha) {
+    return backproject_voxel_impl(ctx, frame, view, corr, alpha, hipMemcpyHostToDevice);
+}
+int stt_backproject_voxel_dev(stt_ctx* ctx, int frame, int view, const float* corr_dev, double alpha) {
+    return backproject_voxel_impl(ctx, frame, view, corr_dev, alpha, hipMemcpyDeviceToDevice);
 }
 
 int stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr, float* contrib, float* wsum) {
@@ -1341,7 +1383,9 @@ int stt_backproject_ray(stt_ctx* ctx, int frame, int view, const float* corr, fl
     return STT_OK;
 }
 
-int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
+namespace {
+
+int fp_vjp_impl(stt_ctx* ctx, int frame, int view, const float* y, float* out, hipMemcpyKind in, hipMemcpyKind back) {
     CTX_CHECK(ctx);
     if (frame < 0 || frame >= ctx->F || view < 0 || view >= ctx->P[frame])
         return fail(STT_ERR_INVALID, "frame/view out of range");
@@ -1352,9 +1396,9 @@ int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
     HIPCHK(acc_mem.alloc((ctx->Vpad + ctx->D) * sizeof(float), ctx->stream));
     acc = (float*)acc_mem.p;
     HIPCHK(hipMemsetAsync(acc, 0, ctx->Vpad * sizeof(float), ctx->stream));
-    HIPCHK(hipMemcpyAsync(acc, out, ctx->V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(acc, out, ctx->V * sizeof(float), in, ctx->stream));
     float* dy = acc + ctx->Vpad;
-    HIPCHK(hipMemcpyAsync(dy, y, ctx->D * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dy, y, ctx->D * sizeof(float), in, ctx->stream));
     WorkItem it = ctx->make_item(frame, view, true);
     it.corr = dy;
     it.contrib = acc;
@@ -1363,9 +1407,18 @@ int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
     if (rc) return rc;
     // A x = sd * sum_s trilinear(x, p_s), so A^T y scatters y*sd with the trilinear weights.
     HIPCHK(launch_fp(FP_MODE_SCATTER, ctx->items_scratch, 1, ctx->vg, ctx->vg.sd, ctx->stream));
-    HIPCHK(hipMemcpyAsync(out, acc, ctx->V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, acc, ctx->V * sizeof(float), back, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
+}
+
+}  // namespace
+
+int stt_fp_vjp(stt_ctx* ctx, int frame, int view, const float* y, float* out) {
+    return fp_vjp_impl(ctx, frame, view, y, out, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+int stt_fp_vjp_dev(stt_ctx* ctx, int frame, int view, const float* y_dev, float* out_dev) {
+    return fp_vjp_impl(ctx, frame, view, y_dev, out_dev, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
 }
 
 int stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n) {
@@ -1396,7 +1449,10 @@ int stt_sart_sweep(stt_ctx* ctx, int frame, const int* order, int n) {
     return STT_OK;
 }
 
-int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* out, int interp) {
+namespace {
+
+int warp_impl(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* out, int interp, hipMemcpyKind in,
+              hipMemcpyKind back) {
     CTX_CHECK(ctx);
     if (!uvw || !src || !out) return fail(STT_ERR_INVALID, "null argument");
     if (interp != STT_INTERP_CUBIC_CLAMPED && interp != STT_INTERP_TRILINEAR)
@@ -1406,13 +1462,22 @@ int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* 
     StreamMem buf_mem;
     HIPCHK(buf_mem.alloc(5 * V * sizeof(float), ctx->stream));
     buf = (float*)buf_mem.p;
-    HIPCHK(hipMemcpyAsync(buf, uvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(buf + 3 * V, src, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(buf, uvw, 3 * V * sizeof(float), in, ctx->stream));
+    HIPCHK(hipMemcpyAsync(buf + 3 * V, src, V * sizeof(float), in, ctx->stream));
     HIPCHK(launch_warp(buf, buf + 3 * V, buf + 4 * V, ctx->vg, dt, interp == STT_INTERP_CUBIC_CLAMPED,
                        ctx->stream));
-    HIPCHK(hipMemcpyAsync(out, buf + 4 * V, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, buf + 4 * V, V * sizeof(float), back, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
+}
+
+}  // namespace
+
+int stt_warp(stt_ctx* ctx, const float* uvw, const float* src, float dt, float* out, int interp) {
+    return warp_impl(ctx, uvw, src, dt, out, interp, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+int stt_warp_dev(stt_ctx* ctx, const float* uvw_dev, const float* src_dev, float dt, float* out_dev, int interp) {
+    return warp_impl(ctx, uvw_dev, src_dev, dt, out_dev, interp, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
 }
 
 int stt_warp_frame(stt_ctx* ctx, int pair, float dt, int interp, float* out) {
@@ -1433,8 +1498,10 @@ int stt_warp_frame(stt_ctx* ctx, int pair, float dt, int interp, float* out) {
     return STT_OK;
 }
 
-int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, const float* g, float* gsrc,
-                 float* guvw) {
+namespace {
+
+int warp_vjp_impl(stt_ctx* ctx, const float* uvw, const float* src, float dt, const float* g, float* gsrc, float* guvw,
+                  hipMemcpyKind in, hipMemcpyKind back) {
     CTX_CHECK(ctx);
     if (!uvw || !src || !g) return fail(STT_ERR_INVALID, "null argument");
     if (!gsrc && !guvw) return STT_OK;
@@ -1444,17 +1511,29 @@ int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, con
     HIPCHK(buf_mem.alloc(9 * V * sizeof(float), ctx->stream));
     buf = (float*)buf_mem.p;
     float *d_uvw = buf, *d_src = buf + 3 * V, *d_g = buf + 4 * V, *d_gsrc = buf + 5 * V, *d_guvw = buf + 6 * V;
-    HIPCHK(hipMemcpyAsync(d_uvw, uvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(d_src, src, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(d_g, g, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    if (gsrc) HIPCHK(hipMemcpyAsync(d_gsrc, gsrc, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    if (guvw) HIPCHK(hipMemcpyAsync(d_guvw, guvw, 3 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_uvw, uvw, 3 * V * sizeof(float), in, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_src, src, V * sizeof(float), in, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_g, g, V * sizeof(float), in, ctx->stream));
+    if (gsrc) HIPCHK(hipMemcpyAsync(d_gsrc, gsrc, V * sizeof(float), in, ctx->stream));
+    if (guvw) HIPCHK(hipMemcpyAsync(d_guvw, guvw, 3 * V * sizeof(float), in, ctx->stream));
     HIPCHK(launch_warp_vjp(d_uvw, d_src, d_g, gsrc ? d_gsrc : nullptr, guvw ? d_guvw : nullptr, ctx->vg, dt,
                            ctx->stream));
-    if (gsrc) HIPCHK(hipMemcpyAsync(gsrc, d_gsrc, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (guvw) HIPCHK(hipMemcpyAsync(guvw, d_guvw, 3 * V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (gsrc) HIPCHK(hipMemcpyAsync(gsrc, d_gsrc, V * sizeof(float), back, ctx->stream));
+    if (guvw) HIPCHK(hipMemcpyAsync(guvw, d_guvw, 3 * V * sizeof(float), back, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return STT_OK;
+}
+
+}  // namespace
+
+int stt_warp_vjp(stt_ctx* ctx, const float* uvw, const float* src, float dt, const float* g, float* gsrc,
+                 float* guvw) {
+    return warp_vjp_impl(ctx, uvw, src, dt, g, gsrc, guvw, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+int stt_warp_vjp_dev(stt_ctx* ctx, const float* uvw_dev, const float* src_dev, float dt, const float* g_dev,
+                     float* gsrc_dev, float* guvw_dev) {
+    return warp_vjp_impl(ctx, uvw_dev, src_dev, dt, g_dev, gsrc_dev, guvw_dev, hipMemcpyDeviceToDevice,
+                         hipMemcpyDeviceToDevice);
 }
 
 int stt_pd_init(stt_ctx* ctx) {

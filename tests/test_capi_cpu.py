@@ -65,6 +65,17 @@ def test_invalid_arguments_are_rejected_before_device_use():
     assert stt.lib.stt_group_iterate(None, 1, 1, None) == stt.STT_ERR_INVALID
     assert stt.lib.stt_group_destroy(None) == stt.STT_OK
     assert stt.lib.stt_warp_frame(None, 0, -1.0, 0, None) == stt.STT_ERR_INVALID
+    # the device-pointer forms check the context before any copy
+    assert stt.lib.stt_set_volume_dev(None, 0, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_get_volume_dev(None, 0, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_set_flow_dev(None, 0, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_set_projections_dev(None, 0, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_forward_project_dev(None, 0, 0, None, None, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_backproject_voxel_dev(None, 0, 0, None, 1.0) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_fp_vjp_dev(None, 0, 0, None, None) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_warp_dev(None, None, None, 1.0, None, 0) == stt.STT_ERR_INVALID
+    assert stt.lib.stt_warp_vjp_dev(None, None, None, 1.0, None, None, None) == stt.STT_ERR_INVALID
+    assert b"null context" in stt.lib.stt_last_error()
 
 
 @pytest.mark.skipif(HAVE_GPU, reason="checks the no-device path")
