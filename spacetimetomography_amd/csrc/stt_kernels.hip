@@ -2391,6 +2391,39 @@ __global__ __launch_bounds__(256) void k_warp_vjp(const float* __restrict__ uvw,
 // residual needs it), G (4: ux uy uz ut), T (4: tau1 tau2 tau3 sigma2).  fp32
 // in the reference's expression order; residual sums in FP64, fixed order.
 
+// Workgroups go to the eight XCDs in turn by linear id, and each XCD has its
+// own L2.  The flow PD kernels read +-1 neighbours across y rows and z
+// slices, so their tiles are remapped: XCD k takes one contiguous run of a
+// tile order and finds most neighbour rows in its own L2.
+//   ZCHUNK = false: x fastest, then y, then z (the primal, whose grid covers
+//     the ROI box);
+//   ZCHUNK = true: chunks of kZc slices, z fastest inside a chunk, then x,
+//     then y (the dual: one 256^2 slice of its 30 planes is about twice an
+//     XCD's L2, so slice-by-slice order loses the z neighbours before the
+//     next slice reads them).
+// Both steps are bijections for any grid size (runs of G/8 or G/8 + 1
+// tiles; the last chunk may be shorter).  `lin` is the plain linear index
+// x + gx (y + gy z) of the tile, which orders the per-block partial sums.
+// Measured at 256^3 on MI355X: dual 435 -> 414 us (fetched bytes 1.19 ->
+// 1.04 of its algorithmic reads), primal 229 -> 218 us; the warp and the
+// gradient kernels ran slower remapped and keep the plain order.
+struct OfTile {
+    unsigned x, y, z, lin;
+};
+template <bool ZCHUNK>
+__device__ __forceinline__ OfTile of_xcd_tile() {
+    constexpr unsigned kXcds = 8, kZc = 8;
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z, G = gx * gy * gz;
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned xcd = L % kXcds, q = G / kXcds, r = G % kXcds;
+    const unsigned T = xcd * q + min(xcd, r) + L / kXcds;
+    if (!ZCHUNK) return OfTile{T % gx, (T / gx) % gy, T / (gx * gy), T};
+    const unsigned zc = T / (gx * gy * kZc), h = min(kZc, gz - kZc * zc);
+    const unsigned rem = T - zc * gx * gy * kZc, t2 = rem / h;
+    const unsigned x = t2 % gx, y = t2 / gx, z = kZc * zc + rem % h;
+    return OfTile{x, y, z, x + gx * (y + gy * z)};
+}
+
 // warp(dim, aabb, V, I2, -dt) with dt = 1 and aabb = [0, ext] (:245): trace
 // from the cell centre in world units, clamped-cubic samples.
 __global__ __launch_bounds__(256) void k_of_warp(const float* __restrict__ v, const float* __restrict__ img,
@@ -2488,10 +2521,10 @@ __device__ __forceinline__ float of_dp(const float* a, size_t q, size_t st, int 
     return i < n - 1 ? d : 0.0f;
 }
 
-__device__ __forceinline__ void of_block_partial(double acc, double* partial) {
+__device__ __forceinline__ void of_block_partial(double acc, double* partial, unsigned tile) {
     __shared__ double red[4];
     const double t = block_sum_f64(acc, red);
-    if (threadIdx.x == 0) partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
+    if (threadIdx.x == 0) partial[tile] = t;
 }
 
 // Primal step on the ROI (:300-340).  MODE 1 keeps K^T y for the next
@@ -2507,9 +2540,10 @@ __global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin
 #pragma clang fp contract(off)
     // the grid covers the ROI's box in y and z; x keeps whole 64-voxel rows
     // so that a wave's loads stay aligned to cache lines
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = roi.y0 + blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = roi.z0 + blockIdx.z;
+    const OfTile tl = of_xcd_tile<false>();
+    const int x = tl.x * 64 + (threadIdx.x & 63);
+    const int y = roi.y0 + tl.y * 4 + (threadIdx.x >> 6);
+    const int z = roi.z0 + tl.z;
     double acc = 0.0;
     if (x < g.nx && y < g.ny && of_in_roi(g, roi, x, y, z)) {
         const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
@@ -2531,7 +2565,7 @@ __global__ __launch_bounds__(256) void k_of_primal(const float* __restrict__ vin
         }
         acc = (double)r;
     }
-    if (MODE == 2) of_block_partial(acc, partial);
+    if (MODE == 2) of_block_partial(acc, partial, tl.lin);
 }
 
 // Dual step on every voxel (:343-480); RES adds the dual residual.
@@ -2550,9 +2584,10 @@ __global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, co
                                                  const float* __restrict__ G, double* __restrict__ partial, VolGeom g,
                                                  float sigma1, float hf, float lambda, int huber) {
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    const OfTile tl = of_xcd_tile<true>();
+    const int x = tl.x * 64 + (threadIdx.x & 63);
+    const int y = tl.y * 4 + (threadIdx.x >> 6);
+    const int z = tl.z;
     double acc = 0.0;
     if (x < g.nx && y < g.ny) {
         const size_t n = (size_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (size_t)g.nx * g.ny;
@@ -2609,7 +2644,7 @@ __global__ __launch_bounds__(256) void k_of_dual(const float* __restrict__ v, co
         if (RES) r = r + fabsf((y4o - y4n) / s2 - k4o + k4);
         acc = (double)r;
     }
-    if (RES) of_block_partial(acc, partial);
+    if (RES) of_block_partial(acc, partial, tl.lin);
 }
 
 // Kx4 of the last iterate with the current image gradient, before the next
